@@ -1,0 +1,104 @@
+/*
+ * ktruss.h — C ABI of the MI355X k-max-truss library (libkc_amd.so).
+ *
+ * Plain pointers and sizes only. Every function returns 0 on success and a
+ * non-zero status on failure; kt_last_error() then describes the failure.
+ * A context owns one HIP stream and its device buffers; calls on one context
+ * must not overlap (the same holds for the reference, whose module-level
+ * device counters make it non-reentrant, src/kc.cu:55-58).
+ *
+ * Which reference interface each entry point replaces:
+ *   kt_solve_tsv            the whole `kc_gpu -f <file>` run: parse + two-pass
+ *                           solve + result line (src/kc_gpu.cpp:209-347)
+ *   kt_solve_lines_host     kmax_truss() called twice on the parsed graph
+ *                           (src/kc_gpu.cpp:129-208, 336-341)
+ *   kt_solve_pairs_host     same, from canonical undirected pairs
+ *   kt_solve_pairs_device   same, input already resident in HBM (no reference
+ *                           counterpart: the reference copies host arrays in
+ *                           every call, src/kc.cu:584-651)
+ *   kt_truss_edges          the kmax-truss edge set, which the reference
+ *                           computes but never emits (src/kc.cu:1236-1242)
+ * The per-stage functions of src/kc_gpu.h are declared in kc_gpu.h (C++ ABI,
+ * same signatures) and in kc_gpu_c.h (C ABI).
+ */
+#ifndef KT_KTRUSS_H
+#define KT_KTRUSS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kt_context kt_context;
+
+typedef struct kt_result {
+  uint32_t kmax;          /* largest k with a non-empty k-truss; 0 if triangle-free */
+  uint32_t max_core;      /* largest k with a non-empty k-core */
+  uint64_t truss_edges;   /* edges in the kmax-truss */
+  uint32_t n;             /* vertex count of the input */
+  uint32_t k_hat;         /* pass 1: kmax of the max-core (a lower bound) */
+  uint64_t m;             /* undirected edges of the input */
+  uint64_t k_hat_edges;
+  uint32_t core_k;        /* pass 2 works on the core_k-core */
+  uint32_t n_core;
+  uint64_t m_core;
+  uint64_t peel_rounds;   /* edge-peel rounds, both passes */
+  uint32_t levels;        /* non-empty truss levels visited */
+  uint32_t lanes;         /* sub-wavefront width chosen for the support count */
+  float ms_total;         /* device time of the whole solve */
+  float ms_build;         /* (a) CSR build */
+  float ms_kcore;         /* (b) vertex peeling, both passes */
+  float ms_pass1;         /* pass 1 on the max-core */
+  float ms_extract;       /* (a1-a3) pass-2 core extraction */
+  float ms_count;         /* (c) pass-2 support count */
+  float ms_peel;          /* (d) pass-2 edge peeling */
+  float ms_count_kernel;  /* device time of the pass-2 support kernel alone */
+} kt_result;
+
+/* Library and device */
+const char* kt_last_error(void);
+int kt_device_count(int* count);
+int kt_create(int device, kt_context** out);
+int kt_destroy(kt_context* ctx);
+
+/* Device memory helpers for callers without their own allocator. */
+int kt_malloc(kt_context* ctx, uint64_t bytes, void** dptr);
+int kt_free(kt_context* ctx, void* dptr);
+int kt_memcpy_h2d(kt_context* ctx, void* dst, const void* src, uint64_t bytes);
+int kt_memcpy_d2h(kt_context* ctx, void* dst, const void* src, uint64_t bytes);
+int kt_synchronize(kt_context* ctx);
+
+/* Whole solves. Pairs are interleaved (u0, v0, u1, v1, ...), u != v, each
+ * undirected edge once, ids < n. Lines follow the reference's input
+ * contract: line i puts xs[i] into row ys[i]; both directions present. */
+int kt_solve_pairs_device(kt_context* ctx, const uint32_t* d_pairs, uint64_t m, uint32_t n,
+                          kt_result* res);
+int kt_solve_pairs_host(kt_context* ctx, const uint32_t* pairs, uint64_t m, uint32_t n,
+                        kt_result* res);
+int kt_solve_lines_host(kt_context* ctx, const uint32_t* xs, const uint32_t* ys, uint64_t L,
+                        uint32_t n, kt_result* res);
+int kt_solve_tsv(kt_context* ctx, const char* path, kt_result* res, double* parse_ms);
+
+/* Survivors of the last solve: original ids, u < v, unordered.
+ * *count receives the truss size; at most cap pairs are written. */
+int kt_truss_edges(kt_context* ctx, uint32_t* pairs, uint64_t cap, uint64_t* count);
+
+/* Host TSV parser (the reference's input contract, src/kc_gpu.cpp:243-312).
+ * Pass NULL buffers to get the line count and n only. */
+int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uint64_t* lines,
+                 uint32_t* n);
+
+/* Tuning: sub-wavefront lanes per row for the support count and the peel
+ * (0 = automatic; 8, 16, 32 or 64). force_edge_count selects the per-edge
+ * support kernel instead of the LDS bitmap one. */
+int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int force_edge_count);
+
+/* Stage access for parity tests: the undirected CSR built by the last solve
+ * (off has n+1 entries, adj 2m). */
+int kt_last_csr(kt_context* ctx, uint64_t* off, uint32_t* adj);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,228 @@
+"""kc_cuda_amd — MI355X-native k-max-truss (drop-in for caixiaofan/KC_CUDA's kc_gpu path).
+
+The compute path is ``lib/libkc_amd.so`` (HIP for gfx950 + C++ host code),
+reached through its C ABI (``include/ktruss.h``). This module is a thin ctypes
+binding over it; it has no CPU fallback and raises when the library is absent.
+
+    from kc_cuda_amd import Context
+    with Context() as ctx:
+        res = ctx.solve_pairs(pairs, n)       # pairs: (m, 2) uint32, u != v
+        print(res.kmax, res.truss_edges)
+
+``kmax_truss(pairs, n)`` mirrors the reference's ``kmax_truss`` two-pass
+driver (src/kc_gpu.cpp:129-208, 336-341) and returns ``(kmax, edges)``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import NativeLibraryMissing  # noqa: F401
+
+__all__ = ["Context", "KtResult", "KtError", "kmax_truss", "native_library", "parse_tsv",
+           "NativeLibraryMissing", "EXPORTS"]
+
+# Every symbol declared in include/ktruss.h and include/kc_gpu_c.h.
+EXPORTS = [
+    "kt_last_error", "kt_device_count", "kt_create", "kt_destroy", "kt_malloc", "kt_free",
+    "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_synchronize", "kt_solve_pairs_device",
+    "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges",
+    "kt_parse_tsv", "kt_set_tuning", "kt_last_csr",
+]
+
+
+class KtError(RuntimeError):
+    pass
+
+
+class KtResult(ctypes.Structure):
+    _fields_ = [
+        ("kmax", ctypes.c_uint32), ("max_core", ctypes.c_uint32),
+        ("truss_edges", ctypes.c_uint64), ("n", ctypes.c_uint32), ("k_hat", ctypes.c_uint32),
+        ("m", ctypes.c_uint64), ("k_hat_edges", ctypes.c_uint64), ("core_k", ctypes.c_uint32),
+        ("n_core", ctypes.c_uint32), ("m_core", ctypes.c_uint64),
+        ("peel_rounds", ctypes.c_uint64), ("levels", ctypes.c_uint32), ("lanes", ctypes.c_uint32),
+        ("ms_total", ctypes.c_float), ("ms_build", ctypes.c_float), ("ms_kcore", ctypes.c_float),
+        ("ms_pass1", ctypes.c_float), ("ms_extract", ctypes.c_float), ("ms_count", ctypes.c_float),
+        ("ms_peel", ctypes.c_float), ("ms_count_kernel", ctypes.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_V = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+
+
+def native_library() -> ctypes.CDLL:
+    lib = _lib.load_device()
+    if getattr(lib, "_kt_typed", False):
+        return lib
+    lib.kt_last_error.restype = ctypes.c_char_p
+    lib.kt_last_error.argtypes = []
+    sig = {
+        "kt_device_count": [ctypes.POINTER(ctypes.c_int)],
+        "kt_create": [ctypes.c_int, ctypes.POINTER(_V)],
+        "kt_destroy": [_V],
+        "kt_malloc": [_V, _U64, ctypes.POINTER(_V)],
+        "kt_free": [_V, _V],
+        "kt_memcpy_h2d": [_V, _V, _V, _U64],
+        "kt_memcpy_d2h": [_V, _V, _V, _U64],
+        "kt_synchronize": [_V],
+        "kt_solve_pairs_device": [_V, _V, _U64, _U32, ctypes.POINTER(KtResult)],
+        "kt_solve_pairs_host": [_V, _V, _U64, _U32, ctypes.POINTER(KtResult)],
+        "kt_solve_lines_host": [_V, _V, _V, _U64, _U32, ctypes.POINTER(KtResult)],
+        "kt_solve_tsv": [_V, ctypes.c_char_p, ctypes.POINTER(KtResult),
+                         ctypes.POINTER(ctypes.c_double)],
+        "kt_truss_edges": [_V, _V, _U64, ctypes.POINTER(_U64)],
+        "kt_parse_tsv": [ctypes.c_char_p, _V, _V, _U64, ctypes.POINTER(_U64),
+                         ctypes.POINTER(_U32)],
+        "kt_set_tuning": [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "kt_last_csr": [_V, _V, _V],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib._kt_typed = True
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = native_library().kt_last_error()
+        raise KtError(f"status {rc}: {msg.decode() if msg else ''}")
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def parse_tsv(path: str | os.PathLike) -> tuple[np.ndarray, np.ndarray, int]:
+    """Parse an edge-list file with the reference's input contract;
+    returns (xs, ys, n) where line i puts xs[i] into row ys[i]."""
+    lib = native_library()
+    lines = _U64()
+    n = _U32()
+    _check(lib.kt_parse_tsv(os.fsencode(str(path)), None, None, 0, ctypes.byref(lines),
+                            ctypes.byref(n)))
+    L = lines.value
+    xs = np.zeros(max(L, 1), dtype=np.uint32)
+    ys = np.zeros(max(L, 1), dtype=np.uint32)
+    _check(lib.kt_parse_tsv(os.fsencode(str(path)), xs.ctypes.data, ys.ctypes.data, L,
+                            ctypes.byref(lines), ctypes.byref(n)))
+    return xs[:L], ys[:L], n.value
+
+
+@dataclass
+class DeviceArray:
+    ctx: "Context"
+    ptr: int
+    nbytes: int
+
+    def free(self) -> None:
+        if self.ptr:
+            _check(native_library().kt_free(self.ctx._h, self.ptr))
+            self.ptr = 0
+
+
+class Context:
+    """One solver context: a HIP stream plus device buffers reused across
+    solves (graphs stay resident until the next solve on this context)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = native_library()
+        h = _V()
+        _check(self._lib.kt_create(device, ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(self._lib.kt_destroy(self._h))
+            self._h = None
+
+    def __enter__(self) -> "Context":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- device memory ------------------------------------------------------
+    def to_device(self, a: np.ndarray) -> DeviceArray:
+        a = np.ascontiguousarray(a)
+        p = _V()
+        _check(self._lib.kt_malloc(self._h, max(a.nbytes, 1), ctypes.byref(p)))
+        d = DeviceArray(self, p.value, a.nbytes)
+        if a.nbytes:
+            _check(self._lib.kt_memcpy_h2d(self._h, p.value, a.ctypes.data, a.nbytes))
+        return d
+
+    # -- solves -------------------------------------------------------------
+    def tune(self, count_lanes: int = 0, peel_lanes: int = 0, force_edge_count: bool = False):
+        _check(self._lib.kt_set_tuning(self._h, count_lanes, peel_lanes, int(force_edge_count)))
+
+    def solve_pairs(self, pairs: np.ndarray, n: int) -> KtResult:
+        pairs = _u32(pairs).reshape(-1, 2)
+        r = KtResult()
+        _check(self._lib.kt_solve_pairs_host(self._h, pairs.ctypes.data if pairs.size else None,
+                                             pairs.shape[0], n, ctypes.byref(r)))
+        return r
+
+    def solve_device(self, dpairs: DeviceArray, m: int, n: int) -> KtResult:
+        r = KtResult()
+        _check(self._lib.kt_solve_pairs_device(self._h, dpairs.ptr, m, n, ctypes.byref(r)))
+        return r
+
+    def solve_lines(self, xs: np.ndarray, ys: np.ndarray, n: int) -> KtResult:
+        xs, ys = _u32(xs), _u32(ys)
+        r = KtResult()
+        _check(self._lib.kt_solve_lines_host(self._h, xs.ctypes.data if xs.size else None,
+                                             ys.ctypes.data if ys.size else None, xs.size, n,
+                                             ctypes.byref(r)))
+        return r
+
+    def solve_tsv(self, path: str | os.PathLike) -> tuple[KtResult, float]:
+        r = KtResult()
+        ms = ctypes.c_double()
+        _check(self._lib.kt_solve_tsv(self._h, os.fsencode(str(path)), ctypes.byref(r),
+                                      ctypes.byref(ms)))
+        return r, ms.value
+
+    def truss_edges(self) -> np.ndarray:
+        """Edges of the kmax-truss of the last solve, original ids, u < v,
+        sorted by (u, v)."""
+        cnt = _U64()
+        _check(self._lib.kt_truss_edges(self._h, None, 0, ctypes.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 2), dtype=np.uint32)
+        _check(self._lib.kt_truss_edges(self._h, out.ctypes.data, cnt.value, ctypes.byref(cnt)))
+        out = out[: cnt.value]
+        order = np.lexsort((out[:, 1], out[:, 0]))
+        return out[order]
+
+    def last_csr(self, n: int, m2: int) -> tuple[np.ndarray, np.ndarray]:
+        off = np.zeros(n + 1, dtype=np.uint64)
+        adj = np.zeros(max(m2, 1), dtype=np.uint32)
+        _check(self._lib.kt_last_csr(self._h, off.ctypes.data, adj.ctypes.data))
+        return off, adj[:m2]
+
+    def synchronize(self) -> None:
+        _check(self._lib.kt_synchronize(self._h))
+
+
+def kmax_truss(pairs: np.ndarray, n: int, device: int = 0) -> tuple[int, int]:
+    """(kmax, edges in the kmax-truss) of an undirected simple graph."""
+    with Context(device) as ctx:
+        r = ctx.solve_pairs(pairs, n)
+        return int(r.kmax), int(r.truss_edges)

@@ -1,0 +1,211 @@
+// extern "C" surface of libkc_amd.so (declared in include/ktruss.h).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "../../include/ktruss.h"
+#include "kt_solver.h"
+#include "kt_tsv.h"
+
+struct kt_context {
+  std::unique_ptr<kt::Solver> solver;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int set_err(const std::string& s, int code = 1) {
+  g_err = s;
+  return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return 0;
+  } catch (const kt::Error& e) {
+    return set_err(e.what, int(e.code) ? int(e.code) : 1);
+  } catch (const std::bad_alloc&) {
+    return set_err("host allocation failed", 2);
+  } catch (const std::exception& e) {
+    return set_err(e.what(), 3);
+  } catch (...) {
+    return set_err("unknown failure", 4);
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* kt_last_error(void) { return g_err.c_str(); }
+
+int kt_device_count(int* count) {
+  return guarded([&] {
+    int c = 0;
+    KT_HIP(hipGetDeviceCount(&c));
+    *count = c;
+  });
+}
+
+int kt_create(int device, kt_context** out) {
+  if (!out) return set_err("kt_create: out is NULL");
+  *out = nullptr;
+  return guarded([&] {
+    auto* c = new kt_context;
+    try {
+      c->solver = std::make_unique<kt::Solver>(device);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int kt_destroy(kt_context* ctx) {
+  return guarded([&] { delete ctx; });
+}
+
+int kt_malloc(kt_context* ctx, uint64_t bytes, void** dptr) {
+  return guarded([&] {
+    KT_HIP(hipSetDevice(ctx->solver->device()));
+    KT_HIP(hipMalloc(dptr, bytes ? bytes : 1));
+  });
+}
+
+int kt_free(kt_context* ctx, void* dptr) {
+  return guarded([&] {
+    KT_HIP(hipSetDevice(ctx->solver->device()));
+    KT_HIP(hipFree(dptr));
+  });
+}
+
+int kt_memcpy_h2d(kt_context* ctx, void* dst, const void* src, uint64_t bytes) {
+  return guarded([&] {
+    KT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->solver->stream()));
+    KT_HIP(hipStreamSynchronize(ctx->solver->stream()));
+  });
+}
+
+int kt_memcpy_d2h(kt_context* ctx, void* dst, const void* src, uint64_t bytes) {
+  return guarded([&] {
+    KT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->solver->stream()));
+    KT_HIP(hipStreamSynchronize(ctx->solver->stream()));
+  });
+}
+
+int kt_synchronize(kt_context* ctx) {
+  return guarded([&] { KT_HIP(hipStreamSynchronize(ctx->solver->stream())); });
+}
+
+int kt_solve_pairs_device(kt_context* ctx, const uint32_t* d_pairs, uint64_t m, uint32_t n,
+                          kt_result* res) {
+  if (!ctx || !res) return set_err("kt_solve_pairs_device: NULL argument");
+  return guarded([&] { ctx->solver->solve_pairs_device(d_pairs, m, n, res); });
+}
+
+int kt_solve_pairs_host(kt_context* ctx, const uint32_t* pairs, uint64_t m, uint32_t n,
+                        kt_result* res) {
+  if (!ctx || !res) return set_err("kt_solve_pairs_host: NULL argument");
+  return guarded([&] {
+    auto& s = *ctx->solver;
+    KT_HIP(hipSetDevice(s.device()));
+    void* d = nullptr;
+    KT_HIP(hipMalloc(&d, sizeof(uint32_t) * 2 * (m ? m : 1)));
+    try {
+      if (m) KT_HIP(hipMemcpyAsync(d, pairs, sizeof(uint32_t) * 2 * m, hipMemcpyHostToDevice, s.stream()));
+      s.solve_pairs_device(static_cast<uint32_t*>(d), m, n, res);
+    } catch (...) {
+      (void)hipStreamSynchronize(s.stream());
+      (void)hipFree(d);
+      throw;
+    }
+    KT_HIP(hipStreamSynchronize(s.stream()));
+    KT_HIP(hipFree(d));
+  });
+}
+
+int kt_solve_lines_host(kt_context* ctx, const uint32_t* xs, const uint32_t* ys, uint64_t L,
+                        uint32_t n, kt_result* res) {
+  if (!ctx || !res) return set_err("kt_solve_lines_host: NULL argument");
+  return guarded([&] {
+    auto& s = *ctx->solver;
+    KT_HIP(hipSetDevice(s.device()));
+    void* d = nullptr;
+    KT_HIP(hipMalloc(&d, sizeof(uint32_t) * 2 * (L ? L : 1)));
+    uint32_t* dx = static_cast<uint32_t*>(d);
+    uint32_t* dy = dx + (L ? L : 1);
+    try {
+      if (L) {
+        KT_HIP(hipMemcpyAsync(dx, xs, sizeof(uint32_t) * L, hipMemcpyHostToDevice, s.stream()));
+        KT_HIP(hipMemcpyAsync(dy, ys, sizeof(uint32_t) * L, hipMemcpyHostToDevice, s.stream()));
+      }
+      s.solve_lines_device(dx, dy, L, n, res);
+    } catch (...) {
+      (void)hipStreamSynchronize(s.stream());
+      (void)hipFree(d);
+      throw;
+    }
+    KT_HIP(hipStreamSynchronize(s.stream()));
+    KT_HIP(hipFree(d));
+  });
+}
+
+int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uint64_t* lines,
+                 uint32_t* n) {
+  return guarded([&] {
+    kt::ParsedLines p = kt::parse_tsv_file(path);
+    if (lines) *lines = p.xs.size();
+    if (n) *n = p.n;
+    if (xs && ys) {
+      const uint64_t take = std::min<uint64_t>(cap, p.xs.size());
+      std::memcpy(xs, p.xs.data(), sizeof(uint32_t) * take);
+      std::memcpy(ys, p.ys.data(), sizeof(uint32_t) * take);
+    }
+  });
+}
+
+int kt_solve_tsv(kt_context* ctx, const char* path, kt_result* res, double* parse_ms) {
+  if (!ctx || !res || !path) return set_err("kt_solve_tsv: NULL argument");
+  kt::ParsedLines p;
+  int rc = guarded([&] {
+    auto t0 = std::chrono::steady_clock::now();
+    p = kt::parse_tsv_file(path);
+    auto t1 = std::chrono::steady_clock::now();
+    if (parse_ms) *parse_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  });
+  if (rc) return rc;
+  return kt_solve_lines_host(ctx, p.xs.data(), p.ys.data(), p.xs.size(), p.n, res);
+}
+
+int kt_truss_edges(kt_context* ctx, uint32_t* pairs, uint64_t cap, uint64_t* count) {
+  if (!ctx) return set_err("kt_truss_edges: NULL context");
+  return guarded([&] {
+    uint64_t c = ctx->solver->truss_edges(pairs, cap);
+    if (count) *count = c;
+  });
+}
+
+int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int force_edge_count) {
+  if (!ctx) return set_err("kt_set_tuning: NULL context");
+  auto ok = [](int l) { return l == 0 || l == 8 || l == 16 || l == 32 || l == 64; };
+  if (!ok(count_lanes) || !ok(peel_lanes)) return set_err("kt_set_tuning: lanes must be 0, 8, 16, 32 or 64");
+  ctx->solver->set_lanes(count_lanes, peel_lanes);
+  ctx->solver->set_force_edge_count(force_edge_count != 0);
+  g_err.clear();
+  return 0;
+}
+
+int kt_last_csr(kt_context* ctx, uint64_t* off, uint32_t* adj) {
+  if (!ctx) return set_err("kt_last_csr: NULL context");
+  return guarded([&] { ctx->solver->csr_to_host(off, adj); });
+}
+
+}  // extern "C"

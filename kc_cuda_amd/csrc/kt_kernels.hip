@@ -1,0 +1,711 @@
+// HIP kernels for the k-max-truss path on CDNA4 (gfx950, wave64).
+//
+// Stage map (reference counterparts, for parity review):
+//   CSR build          src/kc_gpu.cpp:38-65, 243-312 (host parse + init_graph)
+//   vertex peel        src/kc.cu:653-884 (__delete_node / kmax_core_gpu)
+//   rank + orient      src/kc.cu:1311-1580 (count_deg_with_mask, build_nbr_direct)
+//   undirected + ids   src/kc.cu:1583-1673 (build_graph_from_direct)
+//   support count      src/kc.cu:193-339, 505-535 (intersection_direct, __tricount_direct)
+//   edge peel          src/kc.cu:61-191, 886-1056 (intersection_with_mask, __delete_edge, ktruss)
+// The algorithms are restated for wave64 and LDS, not translated: frontier
+// queues replace full-array mask scans, hub rows live in LDS as a bitmap plus
+// a sorted list, and same-row credits are accumulated in LDS.
+#include "kt_kernels.h"
+
+namespace kt {
+namespace {
+
+constexpr int kMaxGrid = 1 << 20;
+
+inline int grid_for(uint64_t items, int per_block = kBlock) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > uint64_t(kMaxGrid)) g = kMaxGrid;
+  return int(g);
+}
+
+__device__ __forceinline__ int lane64() { return int(__lane_id()); }
+
+// Wave-aggregated queue append. Must be reached by every lane of the wave.
+template <typename T>
+__device__ __forceinline__ void wave_append(bool pred, T item, T* q, uint32_t* count) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  const int lane = lane64();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(count, uint32_t(__popcll(mask)));
+  base = __shfl(base, leader);
+  if (pred) q[base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = item;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t group_sum(uint32_t v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return group_sum<64>(v); }
+
+// first index in [lo, hi) with a[idx] >= x
+template <typename P>
+__device__ __forceinline__ uint32_t lower_bound_u32(const P a, uint32_t lo, uint32_t hi, uint32_t x) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------ utilities ---
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = i;
+}
+__global__ void k_add(const uint32_t* a, const uint32_t* b, uint32_t n, uint32_t* out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = a[i] + b[i];
+}
+__global__ void k_gather(const uint32_t* idx, const uint32_t* table, uint32_t n, uint32_t* out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = table[idx[i]];
+}
+
+// ------------------------------------------------------------ (a) CSR ---
+
+__global__ void k_degree_pairs(const uint2* __restrict__ pairs, uint64_t m, uint32_t n,
+                               uint32_t* deg, uint32_t* err) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < m;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint2 p = pairs[i];
+    if (p.x >= n || p.y >= n) {
+      atomicOr(err, 1u);
+      continue;
+    }
+    atomicAdd(&deg[p.x], 1u);
+    atomicAdd(&deg[p.y], 1u);
+  }
+}
+
+__global__ void k_scatter_pairs(const uint2* __restrict__ pairs, uint64_t m, uint32_t n,
+                                unsigned long long* cursor, uint32_t* adj) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < m;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint2 p = pairs[i];
+    if (p.x >= n || p.y >= n) continue;
+    adj[atomicAdd(&cursor[p.x], 1ull)] = p.y;
+    adj[atomicAdd(&cursor[p.y], 1ull)] = p.x;
+  }
+}
+
+__global__ void k_degree_lines(const uint32_t* __restrict__ xs, const uint32_t* __restrict__ ys,
+                               uint64_t L, uint32_t n, uint32_t* deg, uint32_t* err) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < L;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t x = xs[i], y = ys[i];
+    if (x >= n || y >= n) {
+      atomicOr(err, 2u);
+      continue;
+    }
+    atomicAdd(&deg[y], 1u);
+  }
+}
+
+__global__ void k_scatter_lines(const uint32_t* __restrict__ xs, const uint32_t* __restrict__ ys,
+                                uint64_t L, uint32_t n, unsigned long long* cursor, uint32_t* adj) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < L;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t x = xs[i], y = ys[i];
+    if (x >= n || y >= n) continue;
+    adj[atomicAdd(&cursor[y], 1ull)] = x;
+  }
+}
+
+__global__ void k_deg_from_off(const uint64_t* off, uint32_t n, uint32_t* deg) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    deg[v] = uint32_t(off[v + 1] - off[v]);
+}
+
+// ----------------------------------------------------- (b) vertex peel ---
+
+__global__ void k_core_frontier(const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ deg,
+                                uint32_t n, uint32_t k, uint32_t* q, uint32_t* qcount) {
+  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < n;
+       base += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t v = base + threadIdx.x;
+    const bool p = v < n && lvl[v] == kNone && deg[v] < k;
+    wave_append<uint32_t>(p, uint32_t(v), q, qcount);
+  }
+}
+
+__global__ void k_core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t k, uint32_t* lvl) {
+  const uint32_t cnt = *qcount;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+    lvl[q[i]] = k;
+}
+
+// One wave per removed vertex: every still-alive neighbour loses one degree;
+// the decrement that takes it from k to k-1 queues it (exactly once).
+__global__ __launch_bounds__(kBlock) void k_core_process(
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj, const uint32_t* q,
+    const uint32_t* qcount, uint32_t k, const uint32_t* __restrict__ lvl, uint32_t* deg,
+    uint32_t* qnext, uint32_t* qnext_count) {
+  const uint32_t cnt = *qcount;
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < cnt; i += waves) {
+    const uint32_t u = q[i];
+    const uint64_t b = off[u], e = off[u + 1];
+    for (uint64_t j0 = b; j0 < e; j0 += kWave) {
+      const uint64_t j = j0 + lane;
+      bool crossed = false;
+      uint32_t w = 0;
+      if (j < e) {
+        w = adj[j];
+        if (lvl[w] == kNone) crossed = (atomicSub(&deg[w], 1u) == k);
+      }
+      wave_append<uint32_t>(crossed, w, qnext, qnext_count);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_core_alive_deg(const uint64_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ adj,
+                                                           const uint32_t* __restrict__ lvl,
+                                                           uint32_t n, uint32_t* deg) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; v < n; v += waves) {
+    if (lvl[v] != kNone) continue;  // wave-uniform
+    uint32_t c = 0;
+    for (uint64_t j = off[v] + lane; j < off[v + 1]; j += kWave) c += (lvl[adj[j]] == kNone);
+    c = wave_sum(c);
+    if (lane == 0) deg[v] = c;
+  }
+}
+
+__global__ void k_core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t l = lvl[v];
+    if (l != kNone && l > k) lvl[v] = kNone;
+  }
+}
+
+// --------------------------------------------------- (a1-a3) extraction ---
+
+// key = degree-in-core << 32 | (~id): ascending order = cmp_link's
+// (degree ascending, id descending), src/kc_gpu.cpp:27-30.
+__global__ void k_core_collect(const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ deg,
+                               uint32_t n, uint64_t* keys, uint32_t* count) {
+  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < n;
+       base += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t v = base + threadIdx.x;
+    const bool p = v < n && lvl[v] == kNone;
+    const uint64_t key = p ? (uint64_t(deg[v]) << 32 | uint64_t(0xFFFFFFFFu - uint32_t(v))) : 0;
+    wave_append<uint64_t>(p, key, keys, count);
+  }
+}
+
+__global__ void k_assign_rank(const uint64_t* keys, uint32_t nc, uint32_t* to, uint32_t* inv) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nc; r += gridDim.x * blockDim.x) {
+    const uint32_t v = 0xFFFFFFFFu - uint32_t(keys[r]);
+    to[v] = r;
+    inv[r] = v;
+  }
+}
+
+// u -> w iff rank(u) < rank(w) (src/kc.cu:1389, 1511).
+__global__ __launch_bounds__(kBlock) void k_orient_count(const uint64_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ adj,
+                                                         const uint32_t* __restrict__ to,
+                                                         const uint32_t* __restrict__ inv,
+                                                         uint32_t nc, uint32_t* dout) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t r = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < nc; r += waves) {
+    const uint32_t v = inv[r];
+    uint32_t c = 0;
+    for (uint64_t j = off[v] + lane; j < off[v + 1]; j += kWave) {
+      const uint32_t t = to[adj[j]];
+      c += (t != kNone && t > r);
+    }
+    c = wave_sum(c);
+    if (lane == 0) dout[r] = c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_orient_fill(
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+    const uint32_t* __restrict__ to, const uint32_t* __restrict__ inv, uint32_t nc,
+    const uint32_t* __restrict__ off_d, uint32_t* adj_d, uint32_t* src, uint32_t* din) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t r = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < nc; r += waves) {
+    const uint32_t v = inv[r];
+    uint32_t pos = off_d[r];
+    const uint64_t e = off[v + 1];
+    for (uint64_t j0 = off[v]; j0 < e; j0 += kWave) {
+      const uint64_t j = j0 + lane;
+      uint32_t t = kNone;
+      if (j < e) t = to[adj[j]];
+      const bool p = (t != kNone && t > r);
+      const uint64_t mask = __ballot(p);
+      if (p) {
+        const uint32_t at = pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+        adj_d[at] = t;
+        src[at] = r;
+        atomicAdd(&din[t], 1u);
+      }
+      pos += uint32_t(__popcll(mask));
+    }
+  }
+}
+
+// Bitonic sort of one row per workgroup in LDS; rows longer than kSortCap
+// are skipped (the host routes them to a segmented radix sort).
+constexpr uint32_t kSortCap = 8192;
+
+__global__ __launch_bounds__(kBlock) void k_sort_rows(const uint32_t* __restrict__ off,
+                                                      uint32_t nrows, uint32_t* vals) {
+  __shared__ uint32_t buf[kSortCap];
+  for (uint32_t row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const uint32_t b = off[row], len = off[row + 1] - b;
+    if (len < 2 || len > kSortCap) continue;  // block-uniform
+    uint32_t P = 2;
+    while (P < len) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = (i < len) ? vals[b + i] : kNone;
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t t = threadIdx.x; t < P / 2; t += blockDim.x) {
+          const uint32_t lo = 2 * t - (t & (stride - 1));
+          const uint32_t hi = lo + stride;
+          const bool up = ((lo & size) == 0);
+          const uint32_t a = buf[lo], c = buf[hi];
+          if ((a > c) == up) {
+            buf[lo] = c;
+            buf[hi] = a;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) vals[b + i] = buf[i];
+    __syncthreads();
+  }
+}
+
+// Undirected row w = [sources of w's in-edges, ascending] ++ [out(w), ascending]
+// with the oriented edge id of every slot (the reference's `fa`).
+__global__ __launch_bounds__(kBlock) void k_build_undirected(
+    const uint32_t* __restrict__ off_d, const uint32_t* __restrict__ adj_d,
+    const uint32_t* __restrict__ din_off, const uint32_t* __restrict__ in_src,
+    const uint32_t* __restrict__ in_eid, uint32_t nc, const uint32_t* __restrict__ off_u,
+    uint32_t* adj_u, uint32_t* eid_u) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; w < nc; w += waves) {
+    const uint32_t ib = din_off[w], in_n = din_off[w + 1] - ib;
+    const uint32_t ob = off_d[w], out_n = off_d[w + 1] - ob;
+    const uint32_t ub = off_u[w];
+    for (uint32_t i = lane; i < in_n; i += kWave) {
+      adj_u[ub + i] = in_src[ib + i];
+      eid_u[ub + i] = in_eid[ib + i];
+    }
+    for (uint32_t i = lane; i < out_n; i += kWave) {
+      adj_u[ub + in_n + i] = adj_d[ob + i];
+      eid_u[ub + in_n + i] = ob + i;
+    }
+  }
+}
+
+// --------------------------------------------------------- (c) support ---
+
+// One workgroup per vertex u (rank order). out(u) is staged in LDS as a
+// bitmap over ranks plus a sorted list; sub-groups of G lanes stream out(v)
+// for each v in out(u) with coalesced loads and test membership in the
+// bitmap. A hit w closes triangle (u, v, w), u < v < w:
+//   (u,v) and (u,w) are credited in LDS and flushed once per slot,
+//   (v,w) is credited by a global atomic at w's slot in out(v).
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_support_bitmap(const uint32_t* __restrict__ off_d,
+                                                           const uint32_t* __restrict__ adj_d,
+                                                           uint32_t nc, uint32_t words,
+                                                           uint32_t list_cap, int32_t* sup) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* bits = lds;
+  uint32_t* list = bits + words;
+  uint32_t* credit = list + list_cap;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t u = blockIdx.x; u < nc; u += gridDim.x) {
+    const uint32_t beg = off_d[u];
+    const uint32_t du = off_d[u + 1] - beg;
+    if (du < 2) continue;  // block-uniform
+    for (uint32_t i = tid; i < du; i += blockDim.x) {
+      list[i] = adj_d[beg + i];
+      credit[i] = 0;
+    }
+    __syncthreads();
+    const uint32_t wlo = list[0] >> 5, whi = list[du - 1] >> 5;
+    for (uint32_t i = wlo + tid; i <= whi; i += blockDim.x) bits[i] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < du; i += blockDim.x) atomicOr(&bits[list[i] >> 5], 1u << (list[i] & 31));
+    __syncthreads();
+    const uint32_t wmax = list[du - 1];
+    const uint32_t sg = tid / G, lane = tid % G, nsg = blockDim.x / G;
+    for (uint32_t i = sg; i + 1 < du; i += nsg) {
+      const uint32_t v = list[i];
+      const uint32_t vb = off_d[v], ve = off_d[v + 1];
+      uint32_t cnt = 0;
+      for (uint32_t j0 = vb; j0 < ve; j0 += G) {
+        if (adj_d[j0] > wmax) break;  // rows are sorted: nothing further can hit
+        const uint32_t j = j0 + lane;
+        if (j < ve) {
+          const uint32_t w = adj_d[j];
+          if (w <= wmax && ((bits[w >> 5] >> (w & 31)) & 1u)) {
+            ++cnt;
+            const uint32_t p = lower_bound_u32(list, i + 1, du, w);
+            atomicAdd(&credit[p], 1u);
+            atomicAdd(&sup[j], 1);
+          }
+        }
+      }
+      cnt = group_sum<G>(cnt);
+      if (lane == 0 && cnt) atomicAdd(&credit[i], cnt);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < du; i += blockDim.x)
+      if (credit[i]) atomicAdd(&sup[beg + i], int32_t(credit[i]));
+    __syncthreads();
+  }
+}
+
+// General fallback: one sub-group per oriented edge e = u -> v. Candidates
+// are out(u) after e's slot and out(v); the shorter list is streamed, the
+// longer one binary-searched.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_support_edges(const uint32_t* __restrict__ off_d,
+                                                          const uint32_t* __restrict__ adj_d,
+                                                          const uint32_t* __restrict__ src,
+                                                          uint32_t mc, int32_t* sup) {
+  const uint32_t groups = gridDim.x * (blockDim.x / G);
+  const uint32_t lane = threadIdx.x % G;
+  for (uint32_t e = blockIdx.x * (blockDim.x / G) + threadIdx.x / G; e < mc; e += groups) {
+    const uint32_t u = src[e], v = adj_d[e];
+    uint32_t ab = e + 1, ae = off_d[u + 1];
+    uint32_t bb = off_d[v], be = off_d[v + 1];
+    const bool swap = (ae - ab) > (be - bb);
+    if (swap) {
+      uint32_t t = ab; ab = bb; bb = t;
+      t = ae; ae = be; be = t;
+    }
+    uint32_t cnt = 0;
+    for (uint32_t i0 = ab; i0 < ae; i0 += G) {
+      const uint32_t i = i0 + lane;
+      if (i < ae) {
+        const uint32_t x = adj_d[i];
+        const uint32_t p = lower_bound_u32(adj_d, bb, be, x);
+        if (p < be && adj_d[p] == x) {
+          ++cnt;
+          atomicAdd(&sup[i], 1);
+          atomicAdd(&sup[p], 1);
+        }
+      }
+    }
+    cnt = group_sum<G>(cnt);
+    if (lane == 0 && cnt) atomicAdd(&sup[e], int32_t(cnt));
+  }
+}
+
+// ------------------------------------------------------------ (d) peel ---
+
+__global__ void k_edge_frontier(const uint32_t* __restrict__ state, const int32_t* __restrict__ sup,
+                                uint32_t mc, int32_t thr, uint32_t* q, uint32_t* qcount) {
+  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < mc;
+       base += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t e = base + threadIdx.x;
+    const bool p = e < mc && state[e] == kNone && sup[e] < thr;
+    wave_append<uint32_t>(p, uint32_t(e), q, qcount);
+  }
+}
+
+__global__ void k_edge_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t r, uint32_t* state) {
+  const uint32_t cnt = *qcount;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+    state[q[i]] = r;
+}
+
+__device__ __forceinline__ void peel_dec(uint32_t x, int32_t thr, int32_t* sup, uint32_t* qnext,
+                                         uint32_t* qnext_count) {
+  if (atomicSub(&sup[x], 1) == thr) qnext[atomicAdd(qnext_count, 1u)] = x;
+}
+
+// One sub-group per frontier edge e = (u, v). Triangles (u, v, w) whose other
+// two edges are not deleted lose e. Round-synchronous rule: a triangle with
+// two frontier edges is charged to the third edge once, by the frontier edge
+// of smaller id; a triangle with three frontier edges charges nobody. The
+// reference reaches the same sums with doubled supports and a `2 - #frontier`
+// decrement (src/kc.cu:122-146).
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_peel(
+    const uint32_t* __restrict__ off_u, const uint32_t* __restrict__ adj_u,
+    const uint32_t* __restrict__ eid_u, const uint32_t* __restrict__ src,
+    const uint32_t* __restrict__ dst, const uint32_t* __restrict__ q, const uint32_t* qcount,
+    uint32_t r, int32_t thr, const uint32_t* __restrict__ state, int32_t* sup, uint32_t* qnext,
+    uint32_t* qnext_count) {
+  const uint32_t cnt = *qcount;
+  const uint32_t groups = gridDim.x * (blockDim.x / G);
+  const uint32_t lane = threadIdx.x % G;
+  for (uint32_t qi = blockIdx.x * (blockDim.x / G) + threadIdx.x / G; qi < cnt; qi += groups) {
+    const uint32_t e = q[qi];
+    const uint32_t u = src[e], v = dst[e];
+    uint32_t ab = off_u[u], ae = off_u[u + 1];
+    uint32_t bb = off_u[v], be = off_u[v + 1];
+    if ((ae - ab) > (be - bb)) {
+      uint32_t t = ab; ab = bb; bb = t;
+      t = ae; ae = be; be = t;
+    }
+    for (uint32_t i0 = ab; i0 < ae; i0 += G) {
+      const uint32_t i = i0 + lane;
+      if (i >= ae) continue;
+      const uint32_t x = adj_u[i];
+      const uint32_t p = lower_bound_u32(adj_u, bb, be, x);
+      if (p >= be || adj_u[p] != x) continue;
+      const uint32_t ea = eid_u[i], eb = eid_u[p];
+      const uint32_t ma = state[ea], mb = state[eb];
+      if (ma < r || mb < r) continue;  // a side was deleted in an earlier round
+      const bool ina = (ma == r), inb = (mb == r);
+      if (!ina && !inb) {
+        peel_dec(ea, thr, sup, qnext, qnext_count);
+        peel_dec(eb, thr, sup, qnext, qnext_count);
+      } else if (ina && !inb) {
+        if (e < ea) peel_dec(eb, thr, sup, qnext, qnext_count);
+      } else if (!ina && inb) {
+        if (e < eb) peel_dec(ea, thr, sup, qnext, qnext_count);
+      }
+    }
+  }
+}
+
+__global__ void k_min_support(const uint32_t* __restrict__ state, const int32_t* __restrict__ sup,
+                              uint32_t mc, uint32_t* out) {
+  uint32_t best = kNone;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < mc; e += gridDim.x * blockDim.x)
+    if (state[e] == kNone) best = min(best, uint32_t(sup[e]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) best = min(best, uint32_t(__shfl_xor(best, o)));
+  if (lane64() == 0 && best != kNone) atomicMin(out, best);
+}
+
+__global__ void k_alive_pairs(const uint32_t* __restrict__ state, uint32_t mc, uint32_t floor,
+                              const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                              const uint32_t* __restrict__ inv, uint2* out, uint32_t* count) {
+  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < mc;
+       base += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t e = base + threadIdx.x;
+    bool p = false;
+    uint2 pr = make_uint2(0, 0);
+    if (e < mc) {
+      const uint32_t s = state[e];
+      p = (s == kNone) || (s > floor);
+      if (p) {
+        const uint32_t a = inv[src[e]], b = inv[dst[e]];
+        pr = make_uint2(min(a, b), max(a, b));
+      }
+    }
+    wave_append<uint2>(p, pr, out, count);
+  }
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- wrappers ---
+
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (n) k_fill_u32<<<grid_for(n), kBlock, 0, s>>>(p, n, v);
+}
+void fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
+  if (n) k_fill_u64<<<grid_for(n), kBlock, 0, s>>>(p, n, v);
+}
+void iota_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+  if (n) k_iota<<<grid_for(n), kBlock, 0, s>>>(p, n);
+}
+void add_offsets(const uint32_t* a, const uint32_t* b, uint32_t n, uint32_t* out, hipStream_t s) {
+  if (n) k_add<<<grid_for(n), kBlock, 0, s>>>(a, b, n, out);
+}
+void gather_u32(const uint32_t* idx, const uint32_t* table, uint32_t n, uint32_t* out, hipStream_t s) {
+  if (n) k_gather<<<grid_for(n), kBlock, 0, s>>>(idx, table, n, out);
+}
+
+void csr_degree_pairs(const uint32_t* pairs, uint64_t m, uint32_t n, uint32_t* deg, Ctl* ctl,
+                      hipStream_t s) {
+  if (m) k_degree_pairs<<<grid_for(m), kBlock, 0, s>>>(reinterpret_cast<const uint2*>(pairs), m, n,
+                                                      deg, &ctl->err);
+}
+void csr_degree_lines(const uint32_t* xs, const uint32_t* ys, uint64_t L, uint32_t n, uint32_t* deg,
+                      Ctl* ctl, hipStream_t s) {
+  if (L) k_degree_lines<<<grid_for(L), kBlock, 0, s>>>(xs, ys, L, n, deg, &ctl->err);
+}
+void csr_scatter_pairs(const uint32_t* pairs, uint64_t m, uint32_t n, uint64_t* cursor, uint32_t* adj,
+                       hipStream_t s) {
+  if (m)
+    k_scatter_pairs<<<grid_for(m), kBlock, 0, s>>>(reinterpret_cast<const uint2*>(pairs), m, n,
+                                                   reinterpret_cast<unsigned long long*>(cursor), adj);
+}
+void csr_scatter_lines(const uint32_t* xs, const uint32_t* ys, uint64_t L, uint32_t n, uint64_t* cursor,
+                       uint32_t* adj, hipStream_t s) {
+  if (L)
+    k_scatter_lines<<<grid_for(L), kBlock, 0, s>>>(xs, ys, L, n,
+                                                   reinterpret_cast<unsigned long long*>(cursor), adj);
+}
+void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStream_t s) {
+  if (n) k_deg_from_off<<<grid_for(n), kBlock, 0, s>>>(off, n, deg);
+}
+
+void core_frontier_init(const uint32_t* lvl, const uint32_t* deg, uint32_t n, uint32_t k, uint32_t* q,
+                        uint32_t* qcount, hipStream_t s) {
+  if (n) k_core_frontier<<<grid_for(n), kBlock, 0, s>>>(lvl, deg, n, k, q, qcount);
+}
+void core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t k, uint32_t* lvl,
+                hipStream_t s) {
+  if (maxq) k_core_stamp<<<grid_for(maxq), kBlock, 0, s>>>(q, qcount, k, lvl);
+}
+void core_process(const uint64_t* off, const uint32_t* adj, const uint32_t* q, const uint32_t* qcount,
+                  uint32_t maxq, uint32_t k, uint32_t* lvl, uint32_t* deg, uint32_t* qnext,
+                  uint32_t* qnext_count, hipStream_t s) {
+  if (maxq)
+    k_core_process<<<grid_for(uint64_t(maxq) * kWave), kBlock, 0, s>>>(off, adj, q, qcount, k, lvl,
+                                                                       deg, qnext, qnext_count);
+}
+void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, uint32_t n,
+                        uint32_t* deg, hipStream_t s) {
+  if (n) k_core_alive_deg<<<grid_for(uint64_t(n) * kWave), kBlock, 0, s>>>(off, adj, lvl, n, deg);
+}
+void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s) {
+  if (n) k_core_reopen_above<<<grid_for(n), kBlock, 0, s>>>(lvl, n, k);
+}
+
+void core_collect(const uint32_t* lvl, const uint32_t* deg, uint32_t n, uint64_t* keys, uint32_t* count,
+                  hipStream_t s) {
+  if (n) k_core_collect<<<grid_for(n), kBlock, 0, s>>>(lvl, deg, n, keys, count);
+}
+void core_assign_rank(const uint64_t* keys, uint32_t nc, uint32_t* to, uint32_t* inv, hipStream_t s) {
+  if (nc) k_assign_rank<<<grid_for(nc), kBlock, 0, s>>>(keys, nc, to, inv);
+}
+void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to, const uint32_t* inv,
+                  uint32_t nc, uint32_t* dout, hipStream_t s) {
+  if (nc) k_orient_count<<<grid_for(uint64_t(nc) * kWave), kBlock, 0, s>>>(off, adj, to, inv, nc, dout);
+}
+void orient_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* to, const uint32_t* inv,
+                 uint32_t nc, const uint32_t* off_d, uint32_t* adj_d, uint32_t* src, uint32_t* din,
+                 hipStream_t s) {
+  if (nc)
+    k_orient_fill<<<grid_for(uint64_t(nc) * kWave), kBlock, 0, s>>>(off, adj, to, inv, nc, off_d, adj_d,
+                                                                    src, din);
+}
+void sort_rows(const uint32_t* off, uint32_t nrows, uint32_t* vals, uint32_t max_row, hipStream_t s) {
+  (void)max_row;
+  if (nrows) k_sort_rows<<<nrows < 65536 ? nrows : 65536, kBlock, 0, s>>>(off, nrows, vals);
+}
+void build_undirected(const uint32_t* off_d, const uint32_t* adj_d, const uint32_t* din_off,
+                      const uint32_t* in_src_sorted, const uint32_t* in_eid_sorted, uint32_t nc,
+                      const uint32_t* off_u, uint32_t* adj_u, uint32_t* eid_u, hipStream_t s) {
+  if (nc)
+    k_build_undirected<<<grid_for(uint64_t(nc) * kWave), kBlock, 0, s>>>(
+        off_d, adj_d, din_off, in_src_sorted, in_eid_sorted, nc, off_u, adj_u, eid_u);
+}
+
+static uint32_t bitmap_lds_bytes(uint32_t nc, uint32_t max_dout) {
+  const uint32_t words = (nc + 31) / 32;
+  const uint32_t cap = (max_dout + 63) & ~63u;
+  return 4u * (words + 2u * cap);
+}
+bool support_bitmap_fits(uint32_t nc, uint32_t max_dout) {
+  return bitmap_lds_bytes(nc, max_dout) <= 64u * 1024u;
+}
+void support_bitmap(const uint32_t* off_d, const uint32_t* adj_d, uint32_t nc, uint32_t max_dout,
+                    int lanes, int32_t* sup, Ctl* ctl, hipStream_t s) {
+  (void)ctl;
+  if (!nc) return;
+  const uint32_t words = (nc + 31) / 32;
+  const uint32_t cap = (max_dout + 63) & ~63u;
+  const uint32_t lds = bitmap_lds_bytes(nc, max_dout);
+  const int grid = int(nc < (1u << 20) ? nc : (1u << 20));
+  switch (lanes) {
+    case 8: k_support_bitmap<8><<<grid, kBlock, lds, s>>>(off_d, adj_d, nc, words, cap, sup); break;
+    case 16: k_support_bitmap<16><<<grid, kBlock, lds, s>>>(off_d, adj_d, nc, words, cap, sup); break;
+    case 32: k_support_bitmap<32><<<grid, kBlock, lds, s>>>(off_d, adj_d, nc, words, cap, sup); break;
+    default: k_support_bitmap<64><<<grid, kBlock, lds, s>>>(off_d, adj_d, nc, words, cap, sup); break;
+  }
+}
+void support_edges(const uint32_t* off_d, const uint32_t* adj_d, const uint32_t* src, uint32_t nc,
+                   uint32_t mc, int lanes, int32_t* sup, hipStream_t s) {
+  (void)nc;
+  if (!mc) return;
+  switch (lanes) {
+    case 8: k_support_edges<8><<<grid_for(uint64_t(mc) * 8), kBlock, 0, s>>>(off_d, adj_d, src, mc, sup); break;
+    case 16: k_support_edges<16><<<grid_for(uint64_t(mc) * 16), kBlock, 0, s>>>(off_d, adj_d, src, mc, sup); break;
+    case 32: k_support_edges<32><<<grid_for(uint64_t(mc) * 32), kBlock, 0, s>>>(off_d, adj_d, src, mc, sup); break;
+    default: k_support_edges<64><<<grid_for(uint64_t(mc) * 64), kBlock, 0, s>>>(off_d, adj_d, src, mc, sup); break;
+  }
+}
+
+void edge_frontier_init(const uint32_t* state, const int32_t* sup, uint32_t mc, int32_t thr, uint32_t* q,
+                        uint32_t* qcount, hipStream_t s) {
+  if (mc) k_edge_frontier<<<grid_for(mc), kBlock, 0, s>>>(state, sup, mc, thr, q, qcount);
+}
+void edge_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r, uint32_t* state,
+                hipStream_t s) {
+  if (maxq) k_edge_stamp<<<grid_for(maxq), kBlock, 0, s>>>(q, qcount, r, state);
+}
+void peel_process(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, const uint32_t* src,
+                  const uint32_t* dst, const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r,
+                  int32_t thr, int lanes, uint32_t* state, int32_t* sup, uint32_t* qnext,
+                  uint32_t* qnext_count, hipStream_t s) {
+  if (!maxq) return;
+  switch (lanes) {
+    case 8:
+      k_peel<8><<<grid_for(uint64_t(maxq) * 8), kBlock, 0, s>>>(off_u, adj_u, eid_u, src, dst, q, qcount, r,
+                                                                thr, state, sup, qnext, qnext_count);
+      break;
+    case 16:
+      k_peel<16><<<grid_for(uint64_t(maxq) * 16), kBlock, 0, s>>>(off_u, adj_u, eid_u, src, dst, q, qcount,
+                                                                  r, thr, state, sup, qnext, qnext_count);
+      break;
+    case 32:
+      k_peel<32><<<grid_for(uint64_t(maxq) * 32), kBlock, 0, s>>>(off_u, adj_u, eid_u, src, dst, q, qcount,
+                                                                  r, thr, state, sup, qnext, qnext_count);
+      break;
+    default:
+      k_peel<64><<<grid_for(uint64_t(maxq) * 64), kBlock, 0, s>>>(off_u, adj_u, eid_u, src, dst, q, qcount,
+                                                                  r, thr, state, sup, qnext, qnext_count);
+      break;
+  }
+}
+void edge_min_support(const uint32_t* state, const int32_t* sup, uint32_t mc, uint32_t* out, hipStream_t s) {
+  if (mc) k_min_support<<<grid_for(mc), kBlock, 0, s>>>(state, sup, mc, out);
+}
+void edges_alive_above(const uint32_t* state, uint32_t mc, uint32_t stamp_floor, const uint32_t* src,
+                       const uint32_t* dst, const uint32_t* inv, uint32_t* pairs_out, uint32_t* count,
+                       hipStream_t s) {
+  if (mc)
+    k_alive_pairs<<<grid_for(mc), kBlock, 0, s>>>(state, mc, stamp_floor, src, dst, inv,
+                                                 reinterpret_cast<uint2*>(pairs_out), count);
+}
+
+}  // namespace kt

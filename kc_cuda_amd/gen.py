@@ -1,0 +1,105 @@
+"""Synthetic graph workloads for the k-max-truss path.
+
+Thin ctypes wrapper over ``csrc/ktgen.cpp`` (built into ``lib/libktgen.so``).
+Every generator returns an ``(m, 2)`` uint32 array of canonical undirected
+pairs ``u < v`` sorted by ``(u, v)`` (no self-loops, no duplicates) and the
+vertex count ``n``. ``write_tsv`` emits the input contract of the
+reference's ``kc_gpu -f`` (src/kc_gpu.cpp:243-312): both directions, grouped
+by the second column ascending.
+
+The named workloads mirror the benchmark configurations:
+
+* ``er_clique``   ER G(16384, avg deg 16) + 24-clique, seed 0
+* ``rmat``        RMAT scale 22, ef 16, (.57,.19,.19,.05), relabelled, seed 1
+* ``chung_lu``    power-law gamma 2.1, n 50M, ~500M edges + 60-clique, seed 2
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+
+_u32pp = ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32))
+
+
+def _gen():
+    lib = _lib.load_host("libktgen.so")
+    if not getattr(lib, "_kt_typed", False):
+        lib.ktgen_er_clique.restype = ctypes.c_int64
+        lib.ktgen_er_clique.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32,
+                                        ctypes.c_uint64, _u32pp]
+        lib.ktgen_rmat.restype = ctypes.c_int64
+        lib.ktgen_rmat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
+                                   _u32pp]
+        lib.ktgen_chung_lu.restype = ctypes.c_int64
+        lib.ktgen_chung_lu.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_int64,
+                                       ctypes.c_uint32, ctypes.c_uint64, _u32pp]
+        lib.ktgen_write_tsv.restype = ctypes.c_int64
+        lib.ktgen_write_tsv.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]
+        lib.ktgen_free.argtypes = [ctypes.c_void_p]
+        lib.ktgen_free.restype = None
+        lib._kt_typed = True
+    return lib
+
+
+def _take(lib, m: int, ptr) -> np.ndarray:
+    if m < 0:
+        raise MemoryError("graph generator ran out of memory")
+    try:
+        if m == 0:
+            return np.zeros((0, 2), dtype=np.uint32)
+        view = np.ctypeslib.as_array(ptr, shape=(m * 2,))
+        return view.copy().reshape(m, 2)
+    finally:
+        lib.ktgen_free(ptr)
+
+
+def er_clique(n: int = 16384, avg_deg: float = 16.0, clique: int = 24, seed: int = 0):
+    lib = _gen()
+    p = ctypes.POINTER(ctypes.c_uint32)()
+    m = lib.ktgen_er_clique(n, avg_deg, clique, seed, ctypes.byref(p))
+    return _take(lib, m, p), n
+
+
+def rmat(scale: int = 22, edge_factor: int = 16, a: float = 0.57, b: float = 0.19,
+         c: float = 0.19, relabel: bool = True, clique: int = 0, seed: int = 1):
+    lib = _gen()
+    p = ctypes.POINTER(ctypes.c_uint32)()
+    m = lib.ktgen_rmat(scale, edge_factor, a, b, c, int(relabel), clique, seed, ctypes.byref(p))
+    return _take(lib, m, p), 1 << scale
+
+
+def chung_lu(n: int = 50_000_000, gamma: float = 2.1, samples: int = 560_000_000,
+             clique: int = 60, seed: int = 2):
+    lib = _gen()
+    p = ctypes.POINTER(ctypes.c_uint32)()
+    m = lib.ktgen_chung_lu(n, gamma, samples, clique, seed, ctypes.byref(p))
+    return _take(lib, m, p), n
+
+
+def write_tsv(path: str | os.PathLike, pairs: np.ndarray) -> int:
+    """Write pairs in the reference's TSV contract; returns bytes written."""
+    lib = _gen()
+    pairs = np.ascontiguousarray(pairs, dtype=np.uint32)
+    r = lib.ktgen_write_tsv(os.fsencode(str(path)), pairs.ctypes.data, pairs.shape[0])
+    if r < 0:
+        raise OSError(f"cannot write {path}")
+    return int(r)
+
+
+def workload(name: str):
+    """Named benchmark workloads -> (pairs, n)."""
+    if name == "er16k_clique24":
+        return er_clique(16384, 16.0, 24, 0)
+    if name == "rmat22":
+        return rmat(22, 16, 0.57, 0.19, 0.19, True, 0, 1)
+    if name == "chung_lu_50m":
+        return chung_lu(50_000_000, 2.1, 560_000_000, 60, 2)
+    if name.startswith("rmat"):
+        scale = int(name[4:])
+        return rmat(scale, 16, 0.57, 0.19, 0.19, True, 0, 1)
+    raise KeyError(name)

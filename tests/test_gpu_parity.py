@@ -1,0 +1,156 @@
+"""GPU parity: the HIP path through the C ABI against the oracle and the
+committed golden fixtures. Results must be bit-exact: same kmax, same truss
+size and the same surviving edge set."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, canon_pairs
+from graphs import closed_forms, random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle():
+    from oracle import oracle as O
+    return O
+
+
+@pytest.mark.parametrize("case", closed_forms(), ids=lambda c: c[0])
+def test_closed_forms(ctx, case):
+    name, pairs, n, kmax, cnt = case
+    print(f"[gpu] closed form {name} n={n} m={len(pairs)}", flush=True)
+    r = ctx.solve_pairs(pairs, n)
+    assert (r.kmax, r.truss_edges) == (kmax, cnt)
+    t = ctx.truss_edges()
+    assert t.shape[0] == cnt
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_small_vs_oracle(ctx, seed):
+    O = _oracle()
+    n = [20, 40, 80, 150, 300, 600][seed % 6]
+    p = [0.3, 0.2, 0.1, 0.05, 0.03, 0.02][seed % 6]
+    pairs = random_graph(n, p, seed, cliques=(seed % 5 + 3,))
+    print(f"[gpu] random seed={seed} n={n} m={len(pairs)}", flush=True)
+    ref = O.solve_pairs(pairs, n, want_edges=True)
+    r = ctx.solve_pairs(pairs, n)
+    assert (r.kmax, r.truss_edges) == (ref.kmax, ref.truss_edges)
+    assert r.max_core == ref.max_core
+    assert r.k_hat == ref.k_hat
+    np.testing.assert_array_equal(ctx.truss_edges(), canon_pairs(ref.truss))
+
+
+def test_config1_er_clique(ctx):
+    from kc_cuda_amd import gen
+    pairs, n = gen.er_clique(16384, 16.0, 24, 0)
+    print(f"[gpu] ER(16384,16)+K24 m={len(pairs)}", flush=True)
+    gold = json.loads((GOLDEN / "kc_cpu_results.json").read_text())["er16k_clique24"]
+    assert len(pairs) == gold["m"]
+    r = ctx.solve_pairs(pairs, n)
+    assert (r.kmax, r.truss_edges) == (gold["kmax"], gold["truss_edges"])
+    ref = _oracle().solve_pairs(pairs, n, want_edges=True)
+    np.testing.assert_array_equal(ctx.truss_edges(), canon_pairs(ref.truss))
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 32, 64])
+@pytest.mark.parametrize("force_edge", [False, True])
+def test_kernel_variants_agree(ctx, lanes, force_edge):
+    from kc_cuda_amd import gen
+    pairs, n = gen.rmat(14, 16, 0.57, 0.19, 0.19, True, 12, 7)
+    print(f"[gpu] rmat14 lanes={lanes} edge={force_edge}", flush=True)
+    ref = _oracle().solve_pairs(pairs, n, want_edges=True)
+    try:
+        ctx.tune(lanes, lanes, force_edge)
+        r = ctx.solve_pairs(pairs, n)
+        assert (r.kmax, r.truss_edges) == (ref.kmax, ref.truss_edges)
+        np.testing.assert_array_equal(ctx.truss_edges(), canon_pairs(ref.truss))
+    finally:
+        ctx.tune(0, 0, False)
+
+
+@pytest.mark.parametrize("scale", [12, 16, 18])
+def test_rmat_vs_golden(ctx, scale):
+    from kc_cuda_amd import gen
+    gold = json.loads((GOLDEN / "kc_cpu_results.json").read_text())[f"rmat{scale}"]
+    pairs, n = gen.rmat(scale)
+    print(f"[gpu] rmat{scale} m={len(pairs)}", flush=True)
+    assert len(pairs) == gold["m"]
+    r = ctx.solve_pairs(pairs, n)
+    assert (r.kmax, r.truss_edges) == (gold["kmax"], gold["truss_edges"])
+    if scale <= 16:
+        ref = _oracle().solve_pairs(pairs, n, want_edges=True)
+        np.testing.assert_array_equal(ctx.truss_edges(), canon_pairs(ref.truss))
+
+
+def test_csr_matches_oracle(ctx):
+    from kc_cuda_amd import gen
+    O = _oracle()
+    pairs, n = gen.rmat(12, 8, 0.57, 0.19, 0.19, True, 0, 5)
+    ctx.solve_pairs(pairs, n)
+    off, adj = ctx.last_csr(n, 2 * len(pairs))
+    ooff, oadj, _ = O.csr_from_pairs(pairs, n)
+    np.testing.assert_array_equal(off, ooff)
+    for v in range(n):  # rows are sets: the device scatter order is free
+        a = np.sort(adj[off[v]:off[v + 1]])
+        b = np.sort(oadj[ooff[v]:ooff[v + 1]])
+        np.testing.assert_array_equal(a, b)
+
+
+def test_tsv_path_and_cli(ctx, tmp_path):
+    from kc_cuda_amd import gen, parse_tsv
+    from kc_cuda_amd._lib import kc_gpu_binary
+    O = _oracle()
+    pairs, n = gen.er_clique(3000, 10.0, 17, 11)
+    path = tmp_path / "g.tsv"
+    gen.write_tsv(path, pairs)
+    xs, ys, nn = parse_tsv(path)
+    oxs, oys = O.parse_tsv_bytes(path.read_bytes())
+    np.testing.assert_array_equal(xs, oxs)
+    np.testing.assert_array_equal(ys, oys)
+    ref = O.solve_pairs(pairs, n)
+    r, _ = ctx.solve_tsv(path)
+    assert (r.kmax, r.truss_edges) == (ref.kmax, ref.truss_edges)
+    out = subprocess.run([str(kc_gpu_binary()), "-f", str(path)], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip().splitlines()[-1] == \
+        f"kmax = {ref.kmax}, Edges in kmax-truss = {ref.truss_edges}."
+
+
+def test_repeat_determinism_and_relabel(ctx):
+    from kc_cuda_amd import gen
+    pairs, n = gen.rmat(15, 16, 0.57, 0.19, 0.19, True, 20, 3)
+    r1 = ctx.solve_pairs(pairs, n)
+    t1 = ctx.truss_edges()
+    r2 = ctx.solve_pairs(pairs, n)
+    t2 = ctx.truss_edges()
+    assert (r1.kmax, r1.truss_edges) == (r2.kmax, r2.truss_edges)
+    np.testing.assert_array_equal(t1, t2)
+    perm = np.random.default_rng(9).permutation(n).astype(np.uint32)
+    r3 = ctx.solve_pairs(canon_pairs(perm[pairs]), n)
+    assert (r3.kmax, r3.truss_edges) == (r1.kmax, r1.truss_edges)
+    inv = np.argsort(perm).astype(np.uint32)
+    np.testing.assert_array_equal(canon_pairs(inv[ctx.truss_edges()]), t1)
+
+
+def test_device_resident_input(ctx):
+    from kc_cuda_amd import gen
+    pairs, n = gen.rmat(13)
+    d = ctx.to_device(np.ascontiguousarray(pairs))
+    try:
+        r = ctx.solve_device(d, len(pairs), n)
+    finally:
+        d.free()
+    r2 = ctx.solve_pairs(pairs, n)
+    assert (r.kmax, r.truss_edges) == (r2.kmax, r2.truss_edges)
+
+
+def test_bad_vertex_id_raises(ctx):
+    from kc_cuda_amd import KtError
+    pairs = np.array([[0, 1], [1, 7]], dtype=np.uint32)
+    with pytest.raises(KtError):
+        ctx.solve_pairs(pairs, 5)
