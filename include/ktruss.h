@@ -56,6 +56,18 @@ typedef struct kt_result {
   float ms_count_kernel;  /* device time of the pass-2 support kernel alone */
 } kt_result;
 
+/* Byte-model terms of the pass-2 core of the last solve: C is the
+ * core_k-core, C+ its rank orientation, W(S) = sum over oriented edges of S
+ * of d+(u) + d+(v), T the kmax-truss. */
+typedef struct kt_stats {
+  uint32_t n_core;
+  uint32_t max_out_degree;
+  uint64_t m_core;
+  uint64_t w_core;      /* W(C+) */
+  uint64_t w_peeled;    /* W(C+ \ T+) */
+  uint64_t truss_edges;
+} kt_stats;
+
 /* Library and device */
 const char* kt_last_error(void);
 int kt_device_count(int* count);
@@ -93,6 +105,9 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
  * (0 = automatic; 8, 16, 32 or 64). force_edge_count selects the per-edge
  * support kernel instead of the LDS bitmap one. */
 int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int force_edge_count);
+
+/* Byte-model terms of the last solve (one extra device pass, untimed). */
+int kt_last_stats(kt_context* ctx, kt_stats* out);
 
 /* Stage access for parity tests: the undirected CSR built by the last solve
  * (off has n+1 entries, adj 2m). */

@@ -31,7 +31,7 @@ EXPORTS = [
     "kt_last_error", "kt_device_count", "kt_create", "kt_destroy", "kt_malloc", "kt_free",
     "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_synchronize", "kt_solve_pairs_device",
     "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges",
-    "kt_parse_tsv", "kt_set_tuning", "kt_last_csr",
+    "kt_parse_tsv", "kt_set_tuning", "kt_last_csr", "kt_last_stats",
 ]
 
 
@@ -50,6 +50,15 @@ class KtResult(ctypes.Structure):
         ("ms_pass1", ctypes.c_float), ("ms_extract", ctypes.c_float), ("ms_count", ctypes.c_float),
         ("ms_peel", ctypes.c_float), ("ms_count_kernel", ctypes.c_float),
     ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class KtStats(ctypes.Structure):
+    _fields_ = [("n_core", ctypes.c_uint32), ("max_out_degree", ctypes.c_uint32),
+                ("m_core", ctypes.c_uint64), ("w_core", ctypes.c_uint64),
+                ("w_peeled", ctypes.c_uint64), ("truss_edges", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -85,6 +94,7 @@ def native_library() -> ctypes.CDLL:
                          ctypes.POINTER(_U32)],
         "kt_set_tuning": [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "kt_last_csr": [_V, _V, _V],
+        "kt_last_stats": [_V, ctypes.POINTER(KtStats)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -210,6 +220,12 @@ class Context:
         out = out[: cnt.value]
         order = np.lexsort((out[:, 1], out[:, 0]))
         return out[order]
+
+    def stats(self) -> KtStats:
+        """Byte-model terms W(C+), W(C+ minus T+) of the last solve."""
+        s = KtStats()
+        _check(self._lib.kt_last_stats(self._h, ctypes.byref(s)))
+        return s
 
     def last_csr(self, n: int, m2: int) -> tuple[np.ndarray, np.ndarray]:
         off = np.zeros(n + 1, dtype=np.uint64)

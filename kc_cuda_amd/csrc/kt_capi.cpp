@@ -203,6 +203,11 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int force_ed
   return 0;
 }
 
+int kt_last_stats(kt_context* ctx, kt_stats* out) {
+  if (!ctx || !out) return set_err("kt_last_stats: NULL argument");
+  return guarded([&] { ctx->solver->last_stats(out); });
+}
+
 int kt_last_csr(kt_context* ctx, uint64_t* off, uint32_t* adj) {
   if (!ctx) return set_err("kt_last_csr: NULL context");
   return guarded([&] { ctx->solver->csr_to_host(off, adj); });

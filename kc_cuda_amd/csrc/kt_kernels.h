@@ -82,4 +82,7 @@ void edges_alive_above(const uint32_t* state, uint32_t mc, uint32_t stamp_floor,
                        const uint32_t* src, const uint32_t* dst, const uint32_t* inv,
                        uint32_t* pairs_out, uint32_t* count, hipStream_t s);
 
+void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, const uint32_t* state,
+             uint32_t mc, uint32_t floor, uint64_t* out2, hipStream_t s);
+
 }  // namespace kt

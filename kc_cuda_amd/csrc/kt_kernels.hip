@@ -532,9 +532,40 @@ __global__ void k_alive_pairs(const uint32_t* __restrict__ state, uint32_t mc, u
   }
 }
 
+// Byte-model terms of the last core: W(C+) = sum over oriented edges of
+// d+(u) + d+(v), and the same sum over the edges outside the kmax-truss.
+__global__ void k_w_stats(const uint32_t* __restrict__ off_d, const uint32_t* __restrict__ src,
+                          const uint32_t* __restrict__ dst, const uint32_t* __restrict__ state,
+                          uint32_t mc, uint32_t floor, unsigned long long* out) {
+  unsigned long long w_all = 0, w_out = 0;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < mc; e += gridDim.x * blockDim.x) {
+    const uint32_t u = src[e], v = dst[e];
+    const unsigned long long w = (off_d[u + 1] - off_d[u]) + (off_d[v + 1] - off_d[v]);
+    w_all += w;
+    const uint32_t s = state[e];
+    if (!(s == kNone || s > floor)) w_out += w;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    w_all += __shfl_xor(w_all, o);
+    w_out += __shfl_xor(w_out, o);
+  }
+  if (lane64() == 0) {
+    atomicAdd(&out[0], w_all);
+    atomicAdd(&out[1], w_out);
+  }
+}
+
 }  // namespace
 
 // --------------------------------------------------------------- wrappers ---
+
+void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, const uint32_t* state,
+             uint32_t mc, uint32_t floor, uint64_t* out2, hipStream_t s) {
+  if (mc)
+    k_w_stats<<<grid_for(mc), kBlock, 0, s>>>(off_d, src, dst, state, mc, floor,
+                                              reinterpret_cast<unsigned long long*>(out2));
+}
 
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
   if (n) k_fill_u32<<<grid_for(n), kBlock, 0, s>>>(p, n, v);

@@ -46,6 +46,8 @@ class Solver {
   // Copies the survivors of the last solve (original ids, u < v) to host.
   uint64_t truss_edges(uint32_t* host_pairs, uint64_t cap);
 
+  // Byte-model terms of the last pass-2 core.
+  void last_stats(kt_stats* out);
   // Stage access for parity tests: device CSR of the last solve.
   void csr_to_host(uint64_t* off, uint32_t* adj);
   void set_lanes(int count_lanes, int peel_lanes) { lanes_count_ = count_lanes; lanes_peel_ = peel_lanes; }
@@ -97,6 +99,7 @@ class Solver {
   DevBuf temp_;
   CoreView last_core_{};
   uint32_t last_floor_ = 0;
+  uint64_t last_truss_ = 0;
   bool have_truss_ = false;
 };
 

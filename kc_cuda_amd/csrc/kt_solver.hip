@@ -493,8 +493,30 @@ void Solver::solve_csr(kt_result* res) {
   (void)hipEventDestroy(e_end);
   last_core_ = cv;
   last_floor_ = p2.stamp_floor;
+  last_truss_ = res->truss_edges;
   have_truss_ = true;
   if (p2.kmax == 0) last_core_.mc = 0;
+}
+
+void Solver::last_stats(kt_stats* out) {
+  if (!have_truss_) fail("no solve has completed on this context");
+  std::memset(out, 0, sizeof(*out));
+  const CoreView& cv = last_core_;
+  out->n_core = cv.nc;
+  out->m_core = cv.mc;
+  out->max_out_degree = cv.max_dout;
+  out->truss_edges = last_truss_;
+  if (cv.mc == 0) return;
+  DevBuf tmp;
+  tmp.ensure(16);
+  fill_u64(tmp.as<uint64_t>(), 2, 0, stream_);
+  w_stats(cv.off_d, cv.src, cv.adj_d, cv.state, cv.mc, last_floor_, tmp.as<uint64_t>(), stream_);
+  uint64_t h[2];
+  KT_HIP(hipMemcpyAsync(h, tmp.p, 16, hipMemcpyDeviceToHost, stream_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  tmp.release();
+  out->w_core = h[0];
+  out->w_peeled = h[1];
 }
 
 uint64_t Solver::truss_edges(uint32_t* host_pairs, uint64_t cap) {

@@ -262,19 +262,40 @@ int64_t ktgen_write_tsv(const char* path, const uint32_t* pairs, int64_t m) {
   __gnu_parallel::sort(lines.begin(), lines.end());
   FILE* f = std::fopen(path, "wb");
   if (!f) return -1;
-  std::vector<char> buf(1 << 22);
-  size_t pos = 0;
+  // format blocks of lines in parallel, write them in order
+  const size_t L = lines.size(), block = 1 << 20;
+  const size_t nb = (L + block - 1) / block;
   int64_t total = 0;
-  for (uint64_t key : lines) {
-    if (pos + 32 > buf.size()) {
-      if (std::fwrite(buf.data(), 1, pos, f) != pos) { std::fclose(f); return -1; }
-      total += int64_t(pos);
-      pos = 0;
+  const size_t batch = 64;
+  std::vector<std::vector<char>> bufs(batch);
+  for (size_t b0 = 0; b0 < nb; b0 += batch) {
+    const size_t b1 = std::min(nb, b0 + batch);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t b = b0; b < b1; ++b) {
+      auto& buf = bufs[b - b0];
+      buf.resize(block * 24);
+      size_t pos = 0;
+      char tmp[12];
+      for (size_t i = b * block; i < std::min(L, (b + 1) * block); ++i) {
+        for (int half = 0; half < 2; ++half) {
+          uint32_t x = half ? uint32_t(lines[i] >> 32) : uint32_t(lines[i]);
+          int t = 0;
+          do { tmp[t++] = char('0' + x % 10); x /= 10; } while (x);
+          while (t) buf[pos++] = tmp[--t];
+          buf[pos++] = half ? '\n' : '\t';
+        }
+      }
+      buf.resize(pos);
     }
-    pos += size_t(std::snprintf(buf.data() + pos, 32, "%u\t%u\n", uint32_t(key), uint32_t(key >> 32)));
+    for (size_t b = b0; b < b1; ++b) {
+      auto& buf = bufs[b - b0];
+      if (!buf.empty() && std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) {
+        std::fclose(f);
+        return -1;
+      }
+      total += int64_t(buf.size());
+    }
   }
-  if (pos && std::fwrite(buf.data(), 1, pos, f) != pos) { std::fclose(f); return -1; }
-  total += int64_t(pos);
   if (std::fclose(f) != 0) return -1;
   return total;
 }
