@@ -102,9 +102,13 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
                  uint32_t* n);
 
 /* Tuning: sub-wavefront lanes per row for the support count and the peel
- * (0 = automatic; 8, 16, 32 or 64). force_edge_count selects the per-edge
- * support kernel instead of the LDS bitmap one. */
-int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int force_edge_count);
+ * (0 = automatic; 8, 16, 32 or 64) and kernel-variant flags:
+ * KT_FORCE_EDGE_COUNT selects the per-edge support kernel instead of the LDS
+ * bitmap one, KT_FORCE_ATOMIC_BUILD the atomic CSR build instead of the
+ * bucketed one. Every variant computes the same result. */
+#define KT_FORCE_EDGE_COUNT 1
+#define KT_FORCE_ATOMIC_BUILD 2
+int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
 
 /* Byte-model terms of the last solve (one extra device pass, untimed). */
 int kt_last_stats(kt_context* ctx, kt_stats* out);

@@ -180,8 +180,10 @@ class Context:
         return d
 
     # -- solves -------------------------------------------------------------
-    def tune(self, count_lanes: int = 0, peel_lanes: int = 0, force_edge_count: bool = False):
-        _check(self._lib.kt_set_tuning(self._h, count_lanes, peel_lanes, int(force_edge_count)))
+    def tune(self, count_lanes: int = 0, peel_lanes: int = 0, force_edge_count: bool = False,
+             force_atomic_build: bool = False):
+        flags = (1 if force_edge_count else 0) | (2 if force_atomic_build else 0)
+        _check(self._lib.kt_set_tuning(self._h, count_lanes, peel_lanes, flags))
 
     def solve_pairs(self, pairs: np.ndarray, n: int) -> KtResult:
         pairs = _u32(pairs).reshape(-1, 2)

@@ -193,12 +193,13 @@ int kt_truss_edges(kt_context* ctx, uint32_t* pairs, uint64_t cap, uint64_t* cou
   });
 }
 
-int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int force_edge_count) {
+int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
   if (!ctx) return set_err("kt_set_tuning: NULL context");
   auto ok = [](int l) { return l == 0 || l == 8 || l == 16 || l == 32 || l == 64; };
   if (!ok(count_lanes) || !ok(peel_lanes)) return set_err("kt_set_tuning: lanes must be 0, 8, 16, 32 or 64");
   ctx->solver->set_lanes(count_lanes, peel_lanes);
-  ctx->solver->set_force_edge_count(force_edge_count != 0);
+  ctx->solver->set_force_edge_count((flags & KT_FORCE_EDGE_COUNT) != 0);
+  ctx->solver->set_force_atomic_build((flags & KT_FORCE_ATOMIC_BUILD) != 0);
   g_err.clear();
   return 0;
 }

@@ -20,6 +20,20 @@ void csr_scatter_lines(const uint32_t* xs, const uint32_t* ys, uint64_t L, uint3
                        uint64_t* cursor, uint32_t* adj, hipStream_t s);
 void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStream_t s);
 
+// Partitioned build: entries bucketed by row >> S (B buckets, T tiles of
+// `tile` items), then rows built per bucket in LDS. mode 0: a = interleaved
+// pairs (both directions); mode 1: a = xs (columns), b = ys (rows).
+// hist has B*T entries (bucket-major); hoff is its exclusive scan (B*T+1).
+bool csr_partition_plan(uint32_t n, uint32_t* S, uint32_t* B);
+void csr_part_hist(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile,
+                   uint32_t n, uint32_t S, uint32_t B, uint32_t T, uint32_t* hist, uint32_t* err,
+                   hipStream_t s);
+void csr_part_scatter(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile,
+                      uint32_t n, uint32_t S, uint32_t B, uint32_t T, const uint64_t* hoff,
+                      uint32_t* part, hipStream_t s);
+void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, uint32_t B, uint32_t T, uint32_t S,
+                     uint32_t n, uint64_t* off, uint32_t* deg, uint32_t* adj, hipStream_t s);
+
 // (b) vertex peeling (k-core). lvl[v] == kNone: alive; otherwise the
 //     threshold at which v was removed. deg holds alive-neighbour counts.
 void core_frontier_init(const uint32_t* lvl, const uint32_t* deg, uint32_t n, uint32_t k,
@@ -36,10 +50,19 @@ void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t
 void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s);
 
 // (a1-a3) core subgraph extraction.
-void core_collect(const uint32_t* lvl, const uint32_t* deg, uint32_t n, uint64_t* keys,
-                  uint32_t* count, hipStream_t s);
-void core_assign_rank(const uint64_t* sorted_keys, uint32_t nc, uint32_t* to, uint32_t* inv,
-                      hipStream_t s);
+// key = degree << 32 | ~original id (to_orig may be null: identity), val = local id
+void core_collect(const uint32_t* lvl, const uint32_t* deg, const uint32_t* to_orig, uint32_t n,
+                  uint64_t* keys, uint32_t* vals, uint32_t* count, hipStream_t s);
+void core_assign_rank(const uint64_t* sorted_keys, const uint32_t* sorted_vals, uint32_t nc,
+                      uint32_t* to, uint32_t* inv, uint32_t* inv_orig, hipStream_t s);
+// (b') induced subgraph of the alive vertices
+void alive_list(const uint32_t* lvl, uint32_t n, uint32_t* list, uint32_t* new_id,
+                uint32_t* count, hipStream_t s);
+void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_orig, uint32_t na,
+                 uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s);
+void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, const uint32_t* list,
+              const uint32_t* new_id, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj,
+              hipStream_t s);
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
                   const uint32_t* inv, uint32_t nc, uint32_t* dout, hipStream_t s);
 void orient_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
@@ -82,7 +105,21 @@ void edges_alive_above(const uint32_t* state, uint32_t mc, uint32_t stamp_floor,
                        const uint32_t* src, const uint32_t* dst, const uint32_t* inv,
                        uint32_t* pairs_out, uint32_t* count, hipStream_t s);
 
-void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, const uint32_t* state,
-             uint32_t mc, uint32_t floor, uint64_t* out2, hipStream_t s);
+void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, const uint32_t* del_thr,
+             uint32_t mc, int32_t floor, uint64_t* out2, hipStream_t s);
+
+// (d3) compaction of the live edges and deletion bookkeeping on base ids
+void row_alive_count(const uint32_t* off_d, const uint32_t* state, uint32_t nc, uint32_t* cnt,
+                     hipStream_t s);
+void row_compact(const uint32_t* off_d, const uint32_t* adj_d, const uint32_t* orig,
+                 const int32_t* sup, const uint32_t* state, uint32_t nc, const uint32_t* off2,
+                 uint32_t* adj2, uint32_t* src2, uint32_t* orig2, int32_t* sup2, hipStream_t s);
+void count_in_degrees(const uint32_t* adj, uint32_t m, uint32_t* din, hipStream_t s);
+void edge_stamp_thr(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r,
+                    uint32_t thr, uint32_t* state, const uint32_t* orig, uint32_t* del_thr,
+                    hipStream_t s);
+void truss_pairs(const uint32_t* del_thr, uint32_t m0, int32_t floor, const uint32_t* src0,
+                 const uint32_t* dst0, const uint32_t* inv, uint32_t* out, uint32_t* count,
+                 hipStream_t s);
 
 }  // namespace kt

@@ -136,6 +136,139 @@ __global__ void k_scatter_lines(const uint32_t* __restrict__ xs, const uint32_t*
   }
 }
 
+// Partitioned CSR build. Directed entries (row, col) are first bucketed by
+// row >> S (2^S rows per bucket, small enough for an LDS counter array),
+// then each bucket is turned into its rows by one workgroup counting in LDS.
+// Entry source: Mode 0 = canonical pairs (both directions), Mode 1 = lines
+// (row = ys[i], col = xs[i]).
+template <int Mode>
+struct EntrySrc {
+  const uint32_t* a;
+  const uint32_t* b;
+  uint64_t items;  // pairs or lines
+  __device__ __forceinline__ int per_item() const { return Mode == 0 ? 2 : 1; }
+  __device__ __forceinline__ void get(uint64_t i, uint32_t* rows, uint32_t* cols) const {
+    if (Mode == 0) {
+      const uint2 p = reinterpret_cast<const uint2*>(a)[i];
+      rows[0] = p.x;
+      cols[0] = p.y;
+      rows[1] = p.y;
+      cols[1] = p.x;
+    } else {
+      rows[0] = b[i];
+      cols[0] = a[i];
+    }
+  }
+};
+
+template <int Mode>
+__global__ __launch_bounds__(kBlock) void k_part_hist(EntrySrc<Mode> src, uint64_t tile, uint32_t n,
+                                                      uint32_t S, uint32_t B, uint32_t T, uint32_t* hist,
+                                                      uint32_t* err) {
+  extern __shared__ uint32_t h[];
+  const uint32_t t = blockIdx.x;
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint64_t lo = uint64_t(t) * tile;
+  const uint64_t hi = min(src.items, lo + tile);
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    uint32_t rows[2], cols[2];
+    src.get(i, rows, cols);
+    for (int k = 0; k < src.per_item(); ++k) {
+      if (rows[k] >= n || cols[k] >= n) {
+        atomicOr(err, 1u);
+        continue;
+      }
+      atomicAdd(&h[rows[k] >> S], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[uint64_t(i) * T + t] = h[i];
+}
+
+template <int Mode>
+__global__ __launch_bounds__(kBlock) void k_part_scatter(EntrySrc<Mode> src, uint64_t tile, uint32_t n,
+                                                         uint32_t S, uint32_t B, uint32_t T,
+                                                         const uint64_t* __restrict__ hoff, uint2* part) {
+  extern __shared__ unsigned long long cur[];
+  const uint32_t t = blockIdx.x;
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) cur[i] = hoff[uint64_t(i) * T + t];
+  __syncthreads();
+  const uint64_t lo = uint64_t(t) * tile;
+  const uint64_t hi = min(src.items, lo + tile);
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    uint32_t rows[2], cols[2];
+    src.get(i, rows, cols);
+    for (int k = 0; k < src.per_item(); ++k) {
+      if (rows[k] >= n || cols[k] >= n) continue;
+      const unsigned long long p = atomicAdd(&cur[rows[k] >> S], 1ull);
+      part[p] = make_uint2(rows[k], cols[k]);
+    }
+  }
+}
+
+// One workgroup per bucket: row counts in LDS, block scan, placement.
+constexpr int kBucketBlock = 1024;
+__global__ __launch_bounds__(kBucketBlock) void k_bucket_rows(const uint2* __restrict__ part,
+                                                              const uint64_t* __restrict__ hoff,
+                                                              uint32_t T, uint32_t S, uint32_t n,
+                                                              uint64_t* off, uint32_t* deg, uint32_t* adj) {
+  extern __shared__ uint32_t cnt[];
+  __shared__ uint32_t wsum[kBucketBlock / kWave];
+  const uint32_t b = blockIdx.x;
+  const uint32_t R = 1u << S;
+  const uint32_t r0 = b << S;
+  const uint32_t nrows = min(R, n - r0);
+  const uint64_t beg = hoff[uint64_t(b) * T], end = hoff[uint64_t(b + 1) * T];
+  for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) cnt[i] = 0;
+  __syncthreads();
+  for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&cnt[part[i].x - r0], 1u);
+  __syncthreads();
+  // exclusive scan of cnt[0..R): each thread owns R / blockDim consecutive rows
+  const uint32_t per = R / blockDim.x;
+  const uint32_t base_i = threadIdx.x * per;
+  uint32_t local = 0;
+  for (uint32_t k = 0; k < per; ++k) local += cnt[base_i + k];
+  // wave-inclusive scan of `local`
+  const int lane = lane64();
+  uint32_t inc = local;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
+  }
+  if (lane == kWave - 1) wsum[threadIdx.x / kWave] = inc;
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    uint32_t w = threadIdx.x < blockDim.x / kWave ? wsum[threadIdx.x] : 0;
+    uint32_t wi = w;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(wi, o);
+      if (int(threadIdx.x) >= o) wi += y;
+    }
+    if (threadIdx.x < blockDim.x / kWave) wsum[threadIdx.x] = wi - w;
+  }
+  __syncthreads();
+  uint32_t run = wsum[threadIdx.x / kWave] + inc - local;
+  for (uint32_t k = 0; k < per; ++k) {
+    const uint32_t r = base_i + k;
+    const uint32_t c = cnt[r];
+    if (r < nrows) {
+      off[r0 + r] = beg + run;
+      deg[r0 + r] = c;
+    }
+    cnt[r] = run;  // becomes the placement cursor
+    run += c;
+  }
+  __syncthreads();
+  for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+    const uint2 e = part[i];
+    const uint32_t p = atomicAdd(&cnt[e.x - r0], 1u);
+    adj[beg + p] = e.y;
+  }
+}
+
 __global__ void k_deg_from_off(const uint64_t* off, uint32_t n, uint32_t* deg) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
     deg[v] = uint32_t(off[v + 1] - off[v]);
@@ -211,21 +344,96 @@ __global__ void k_core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k) {
 // key = degree-in-core << 32 | (~id): ascending order = cmp_link's
 // (degree ascending, id descending), src/kc_gpu.cpp:27-30.
 __global__ void k_core_collect(const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ deg,
-                               uint32_t n, uint64_t* keys, uint32_t* count) {
+                               const uint32_t* __restrict__ to_orig, uint32_t n, uint64_t* keys,
+                               uint32_t* vals, uint32_t* count) {
+  const int lane = lane64();
   for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < n;
        base += uint64_t(gridDim.x) * blockDim.x) {
     const uint64_t v = base + threadIdx.x;
     const bool p = v < n && lvl[v] == kNone;
-    const uint64_t key = p ? (uint64_t(deg[v]) << 32 | uint64_t(0xFFFFFFFFu - uint32_t(v))) : 0;
-    wave_append<uint64_t>(p, key, keys, count);
+    const uint64_t mask = __ballot(p);
+    if (mask == 0) continue;  // wave-uniform
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t at = 0;
+    if (lane == leader) at = atomicAdd(count, uint32_t(__popcll(mask)));
+    at = __shfl(at, leader) + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (p) {
+      const uint32_t id = to_orig ? to_orig[v] : uint32_t(v);
+      keys[at] = uint64_t(deg[v]) << 32 | uint64_t(0xFFFFFFFFu - id);
+      vals[at] = uint32_t(v);
+    }
   }
 }
 
-__global__ void k_assign_rank(const uint64_t* keys, uint32_t nc, uint32_t* to, uint32_t* inv) {
+// sorted (key, local id) -> rank maps; inv_orig carries the input's ids
+__global__ void k_assign_rank(const uint64_t* keys, const uint32_t* vals, uint32_t nc, uint32_t* to,
+                              uint32_t* inv, uint32_t* inv_orig) {
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nc; r += gridDim.x * blockDim.x) {
-    const uint32_t v = 0xFFFFFFFFu - uint32_t(keys[r]);
+    const uint32_t v = vals[r];
     to[v] = r;
     inv[r] = v;
+    inv_orig[r] = 0xFFFFFFFFu - uint32_t(keys[r]);
+  }
+}
+
+// ---------------------------------------------- (b') induced subgraph ---
+
+// Induced subgraph of the alive vertices: sub[i] = alive vertex, new_id[v] = i.
+__global__ void k_alive_list(const uint32_t* __restrict__ lvl, uint32_t n, uint32_t* list,
+                             uint32_t* new_id, uint32_t* count) {
+  const int lane = lane64();
+  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < n;
+       base += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t v = base + threadIdx.x;
+    const bool p = v < n && lvl[v] == kNone;
+    const uint64_t mask = __ballot(p);
+    if (mask == 0) continue;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t at = 0;
+    if (lane == leader) at = atomicAdd(count, uint32_t(__popcll(mask)));
+    at = __shfl(at, leader) + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (p) {
+      list[at] = uint32_t(v);
+      new_id[v] = at;
+    }
+  }
+}
+
+__global__ void k_sub_degrees(const uint32_t* __restrict__ list, const uint32_t* __restrict__ deg,
+                              const uint32_t* __restrict__ to_orig, uint32_t na, uint32_t* sub_deg,
+                              uint32_t* sub_orig) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    const uint32_t v = list[i];
+    sub_deg[i] = deg[v];
+    sub_orig[i] = to_orig ? to_orig[v] : v;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ adj,
+                                                     const uint32_t* __restrict__ lvl,
+                                                     const uint32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ new_id, uint32_t na,
+                                                     const uint64_t* __restrict__ sub_off,
+                                                     uint32_t* sub_adj) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < na; i += waves) {
+    const uint32_t v = list[i];
+    uint64_t pos = sub_off[i];
+    const uint64_t e = off[v + 1];
+    for (uint64_t j0 = off[v]; j0 < e; j0 += kWave) {
+      const uint64_t j = j0 + lane;
+      uint32_t w = 0;
+      bool p = false;
+      if (j < e) {
+        w = adj[j];
+        p = lvl[w] == kNone;
+      }
+      const uint64_t mask = __ballot(p);
+      if (p) sub_adj[pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = new_id[w];
+      pos += uint32_t(__popcll(mask));
+    }
   }
 }
 
@@ -269,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_orient_fill(
         const uint32_t at = pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
         adj_d[at] = t;
         src[at] = r;
-        atomicAdd(&din[t], 1u);
+        if (din) atomicAdd(&din[t], 1u);
       }
       pos += uint32_t(__popcll(mask));
     }
@@ -532,18 +740,98 @@ __global__ void k_alive_pairs(const uint32_t* __restrict__ state, uint32_t mc, u
   }
 }
 
+// ---------------------------------------------------- (d3) compaction ---
+
+__global__ __launch_bounds__(kBlock) void k_row_alive_count(const uint32_t* __restrict__ off_d,
+                                                            const uint32_t* __restrict__ state,
+                                                            uint32_t nc, uint32_t* cnt) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t r = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < nc; r += waves) {
+    uint32_t c = 0;
+    for (uint32_t j = off_d[r] + lane; j < off_d[r + 1]; j += kWave) c += (state[j] == kNone);
+    c = wave_sum(c);
+    if (lane == 0) cnt[r] = c;
+  }
+}
+
+// Stable per-row compaction of the live edges (rows stay sorted).
+__global__ __launch_bounds__(kBlock) void k_row_compact(
+    const uint32_t* __restrict__ off_d, const uint32_t* __restrict__ adj_d,
+    const uint32_t* __restrict__ orig, const int32_t* __restrict__ sup,
+    const uint32_t* __restrict__ state, uint32_t nc, const uint32_t* __restrict__ off2,
+    uint32_t* adj2, uint32_t* src2, uint32_t* orig2, int32_t* sup2) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t r = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < nc; r += waves) {
+    uint32_t pos = off2[r];
+    const uint32_t e = off_d[r + 1];
+    for (uint32_t j0 = off_d[r]; j0 < e; j0 += kWave) {
+      const uint32_t j = j0 + lane;
+      const bool p = j < e && state[j] == kNone;
+      const uint64_t mask = __ballot(p);
+      if (p) {
+        const uint32_t at = pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+        adj2[at] = adj_d[j];
+        src2[at] = r;
+        orig2[at] = orig[j];
+        if (sup2) sup2[at] = sup[j];
+      }
+      pos += uint32_t(__popcll(mask));
+    }
+  }
+}
+
+__global__ void k_count_din(const uint32_t* __restrict__ adj, uint32_t m, uint32_t* din) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+    atomicAdd(&din[adj[i]], 1u);
+}
+
+// Frontier stamp that also records, per base edge, the threshold of the
+// level that deleted it.
+__global__ void k_edge_stamp_thr(const uint32_t* q, const uint32_t* qcount, uint32_t r, uint32_t thr,
+                                 uint32_t* state, const uint32_t* __restrict__ orig, uint32_t* del_thr) {
+  const uint32_t cnt = *qcount;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const uint32_t e = q[i];
+    state[e] = r;
+    del_thr[orig[e]] = thr;
+  }
+}
+
+// Survivors of the level with threshold `floor` among the base edges.
+__global__ void k_truss_pairs(const uint32_t* __restrict__ del_thr, uint32_t m0, int32_t floor,
+                              const uint32_t* __restrict__ src0, const uint32_t* __restrict__ dst0,
+                              const uint32_t* __restrict__ inv, uint2* out, uint32_t* count) {
+  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < m0;
+       base += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t e = base + threadIdx.x;
+    bool p = false;
+    uint2 pr = make_uint2(0, 0);
+    if (e < m0) {
+      const uint32_t d = del_thr[e];
+      p = (d == kNone) || (int64_t(d) > int64_t(floor));
+      if (p) {
+        const uint32_t a = inv[src0[e]], b = inv[dst0[e]];
+        pr = make_uint2(min(a, b), max(a, b));
+      }
+    }
+    wave_append<uint2>(p, pr, out, count);
+  }
+}
+
 // Byte-model terms of the last core: W(C+) = sum over oriented edges of
 // d+(u) + d+(v), and the same sum over the edges outside the kmax-truss.
 __global__ void k_w_stats(const uint32_t* __restrict__ off_d, const uint32_t* __restrict__ src,
-                          const uint32_t* __restrict__ dst, const uint32_t* __restrict__ state,
-                          uint32_t mc, uint32_t floor, unsigned long long* out) {
+                          const uint32_t* __restrict__ dst, const uint32_t* __restrict__ del_thr,
+                          uint32_t mc, int32_t floor, unsigned long long* out) {
   unsigned long long w_all = 0, w_out = 0;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < mc; e += gridDim.x * blockDim.x) {
     const uint32_t u = src[e], v = dst[e];
     const unsigned long long w = (off_d[u + 1] - off_d[u]) + (off_d[v + 1] - off_d[v]);
     w_all += w;
-    const uint32_t s = state[e];
-    if (!(s == kNone || s > floor)) w_out += w;
+    const uint32_t d = del_thr[e];
+    if (!(d == kNone || int64_t(d) > int64_t(floor))) w_out += w;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -560,11 +848,34 @@ __global__ void k_w_stats(const uint32_t* __restrict__ off_d, const uint32_t* __
 
 // --------------------------------------------------------------- wrappers ---
 
-void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, const uint32_t* state,
-             uint32_t mc, uint32_t floor, uint64_t* out2, hipStream_t s) {
+void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, const uint32_t* del_thr,
+             uint32_t mc, int32_t floor, uint64_t* out2, hipStream_t s) {
   if (mc)
-    k_w_stats<<<grid_for(mc), kBlock, 0, s>>>(off_d, src, dst, state, mc, floor,
+    k_w_stats<<<grid_for(mc), kBlock, 0, s>>>(off_d, src, dst, del_thr, mc, floor,
                                               reinterpret_cast<unsigned long long*>(out2));
+}
+void row_alive_count(const uint32_t* off_d, const uint32_t* state, uint32_t nc, uint32_t* cnt, hipStream_t s) {
+  if (nc) k_row_alive_count<<<grid_for(uint64_t(nc) * kWave), kBlock, 0, s>>>(off_d, state, nc, cnt);
+}
+void row_compact(const uint32_t* off_d, const uint32_t* adj_d, const uint32_t* orig, const int32_t* sup,
+                 const uint32_t* state, uint32_t nc, const uint32_t* off2, uint32_t* adj2, uint32_t* src2,
+                 uint32_t* orig2, int32_t* sup2, hipStream_t s) {
+  if (nc)
+    k_row_compact<<<grid_for(uint64_t(nc) * kWave), kBlock, 0, s>>>(off_d, adj_d, orig, sup, state, nc, off2,
+                                                                    adj2, src2, orig2, sup2);
+}
+void count_in_degrees(const uint32_t* adj, uint32_t m, uint32_t* din, hipStream_t s) {
+  if (m) k_count_din<<<grid_for(m), kBlock, 0, s>>>(adj, m, din);
+}
+void edge_stamp_thr(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r, uint32_t thr,
+                    uint32_t* state, const uint32_t* orig, uint32_t* del_thr, hipStream_t s) {
+  if (maxq) k_edge_stamp_thr<<<grid_for(maxq), kBlock, 0, s>>>(q, qcount, r, thr, state, orig, del_thr);
+}
+void truss_pairs(const uint32_t* del_thr, uint32_t m0, int32_t floor, const uint32_t* src0,
+                 const uint32_t* dst0, const uint32_t* inv, uint32_t* out, uint32_t* count, hipStream_t s) {
+  if (m0)
+    k_truss_pairs<<<grid_for(m0), kBlock, 0, s>>>(del_thr, m0, floor, src0, dst0, inv,
+                                                 reinterpret_cast<uint2*>(out), count);
 }
 
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
@@ -604,6 +915,41 @@ void csr_scatter_lines(const uint32_t* xs, const uint32_t* ys, uint64_t L, uint3
     k_scatter_lines<<<grid_for(L), kBlock, 0, s>>>(xs, ys, L, n,
                                                    reinterpret_cast<unsigned long long*>(cursor), adj);
 }
+bool csr_partition_plan(uint32_t n, uint32_t* S, uint32_t* B) {
+  // 2^13 rows keep k_bucket_rows' counters at 32 KB of LDS
+  for (uint32_t s = 10; s <= 13; ++s) {
+    const uint64_t b = (uint64_t(n) + (1u << s) - 1) >> s;
+    if (b <= 8192) {
+      *S = s;
+      *B = uint32_t(b ? b : 1);
+      return true;
+    }
+  }
+  return false;
+}
+void csr_part_hist(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile, uint32_t n,
+                   uint32_t S, uint32_t B, uint32_t T, uint32_t* hist, uint32_t* err, hipStream_t s) {
+  const size_t lds = sizeof(uint32_t) * B;
+  if (mode == 0)
+    k_part_hist<0><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, T, hist, err);
+  else
+    k_part_hist<1><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, T, hist, err);
+}
+void csr_part_scatter(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile,
+                      uint32_t n, uint32_t S, uint32_t B, uint32_t T, const uint64_t* hoff, uint32_t* part,
+                      hipStream_t s) {
+  const size_t lds = sizeof(unsigned long long) * B;
+  uint2* p = reinterpret_cast<uint2*>(part);
+  if (mode == 0)
+    k_part_scatter<0><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, T, hoff, p);
+  else
+    k_part_scatter<1><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, T, hoff, p);
+}
+void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, uint32_t B, uint32_t T, uint32_t S, uint32_t n,
+                     uint64_t* off, uint32_t* deg, uint32_t* adj, hipStream_t s) {
+  const size_t lds = sizeof(uint32_t) << S;
+  k_bucket_rows<<<B, kBucketBlock, lds, s>>>(reinterpret_cast<const uint2*>(part), hoff, T, S, n, off, deg, adj);
+}
 void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStream_t s) {
   if (n) k_deg_from_off<<<grid_for(n), kBlock, 0, s>>>(off, n, deg);
 }
@@ -631,12 +977,27 @@ void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s) {
   if (n) k_core_reopen_above<<<grid_for(n), kBlock, 0, s>>>(lvl, n, k);
 }
 
-void core_collect(const uint32_t* lvl, const uint32_t* deg, uint32_t n, uint64_t* keys, uint32_t* count,
-                  hipStream_t s) {
-  if (n) k_core_collect<<<grid_for(n), kBlock, 0, s>>>(lvl, deg, n, keys, count);
+void core_collect(const uint32_t* lvl, const uint32_t* deg, const uint32_t* to_orig, uint32_t n,
+                  uint64_t* keys, uint32_t* vals, uint32_t* count, hipStream_t s) {
+  if (n) k_core_collect<<<grid_for(n), kBlock, 0, s>>>(lvl, deg, to_orig, n, keys, vals, count);
 }
-void core_assign_rank(const uint64_t* keys, uint32_t nc, uint32_t* to, uint32_t* inv, hipStream_t s) {
-  if (nc) k_assign_rank<<<grid_for(nc), kBlock, 0, s>>>(keys, nc, to, inv);
+void core_assign_rank(const uint64_t* keys, const uint32_t* vals, uint32_t nc, uint32_t* to, uint32_t* inv,
+                      uint32_t* inv_orig, hipStream_t s) {
+  if (nc) k_assign_rank<<<grid_for(nc), kBlock, 0, s>>>(keys, vals, nc, to, inv, inv_orig);
+}
+void alive_list(const uint32_t* lvl, uint32_t n, uint32_t* list, uint32_t* new_id, uint32_t* count,
+                hipStream_t s) {
+  if (n) k_alive_list<<<grid_for(n), kBlock, 0, s>>>(lvl, n, list, new_id, count);
+}
+void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_orig, uint32_t na,
+                 uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s) {
+  if (na) k_sub_degrees<<<grid_for(na), kBlock, 0, s>>>(list, deg, to_orig, na, sub_deg, sub_orig);
+}
+void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, const uint32_t* list,
+              const uint32_t* new_id, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, hipStream_t s) {
+  if (na)
+    k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, lvl, list, new_id, na, sub_off,
+                                                                 sub_adj);
 }
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to, const uint32_t* inv,
                   uint32_t nc, uint32_t* dout, hipStream_t s) {

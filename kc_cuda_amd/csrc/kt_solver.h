@@ -1,5 +1,6 @@
 // Host orchestration of the device-resident k-max-truss solve.
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "kt_common.h"
@@ -17,20 +18,49 @@ struct DevBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
+// One graph of the vertex-peel nest. Level 0 is the input CSR; each deeper
+// level is the induced subgraph of the vertices still alive when it was cut,
+// with its own dense ids (to_orig maps them back to input ids).
+struct LevelGraph {
+  uint32_t n = 0;
+  uint64_t slots = 0;
+  uint64_t* off = nullptr;
+  uint32_t* adj = nullptr;
+  uint32_t* deg = nullptr;       // alive-neighbour count of alive vertices
+  uint32_t* lvl = nullptr;       // kNone: alive; else threshold that removed it
+  uint32_t* to_orig = nullptr;   // null on level 0
+  uint32_t k_enter = 0;          // every vertex here has core number >= k_enter
+  std::vector<uint32_t> ok;      // thresholds whose peel left vertices alive
+  DevBuf b_off, b_adj, b_deg, b_lvl, b_orig;
+};
+
 // Core subgraph in rank ids: oriented CSR (rows sorted), undirected CSR with
 // the oriented edge id of every slot, supports and peel state per edge.
+// The base edge set (the extraction's output) is never modified; compaction
+// writes the surviving edges into one of two work sets and `orig` maps each
+// current edge back to its base id, where its deletion threshold is recorded.
 struct CoreView {
   uint32_t nc = 0, mc = 0, max_dout = 0;
-  uint32_t *inv, *off_d, *adj_d, *src, *off_u, *adj_u, *eid_u, *state;
-  int32_t* sup;
+  uint32_t *inv = nullptr, *off_d = nullptr, *adj_d = nullptr, *src = nullptr, *orig = nullptr;
+  uint32_t *off_u = nullptr, *adj_u = nullptr, *eid_u = nullptr, *state = nullptr;
+  int32_t* sup = nullptr;
+  // base set
+  uint32_t mc0 = 0, max_dout0 = 0;
+  uint32_t *off_d0 = nullptr, *adj_d0 = nullptr, *src0 = nullptr, *del_thr = nullptr;
+  int work = -1;  // which work set holds the current edges (-1: base)
 };
 
 struct TrussOutcome {
   uint32_t kmax = 0;
   uint64_t count = 0;
-  uint32_t stamp_floor = 0;  // survivors: state == kNone or state > floor
+  int32_t floor_thr = -1;  // survivors: del_thr == kNone or del_thr > floor_thr
   uint32_t levels = 0;
   uint64_t rounds = 0;
+  uint32_t compactions = 0;
+};
+
+struct EdgeSet {
+  DevBuf off, adj, src, orig, sup;
 };
 
 class Solver {
@@ -45,27 +75,31 @@ class Solver {
                           kt_result* res);
   // Copies the survivors of the last solve (original ids, u < v) to host.
   uint64_t truss_edges(uint32_t* host_pairs, uint64_t cap);
-
   // Byte-model terms of the last pass-2 core.
   void last_stats(kt_stats* out);
   // Stage access for parity tests: device CSR of the last solve.
   void csr_to_host(uint64_t* off, uint32_t* adj);
   void set_lanes(int count_lanes, int peel_lanes) { lanes_count_ = count_lanes; lanes_peel_ = peel_lanes; }
   void set_force_edge_count(bool v) { force_edge_count_ = v; }
+  void set_force_atomic_build(bool v) { force_atomic_build_ = v; }
 
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }
 
-  // stage-level entry points (used by the reference-compatible API)
+  // stages (kt_stages.hip)
   void load_csr_host(uint32_t n, const uint64_t* off, const uint32_t* adj);
-  uint64_t peel_vertices(uint32_t k);
-  uint32_t max_core(uint64_t alive, std::vector<uint32_t>* ok_thresholds);
-  void extract_core(CoreView* cv);
+  uint64_t peel_vertices(LevelGraph& g, uint32_t k);
+  uint32_t max_core();
+  LevelGraph& cut_level(LevelGraph& parent, uint32_t k_enter, uint32_t alive);
+  void core_at(uint32_t k);  // leaves the k-core alive on the deepest usable level
+  void extract_core(LevelGraph& g, CoreView* cv);
+  void build_undirected_view(CoreView* cv);
   void count_support(CoreView* cv, float* ms);
+  // Keep only the live edges (state == kNone) of the current view; with
+  // carry_sup their supports are kept, otherwise recounted exactly.
+  void compact_core(CoreView* cv, bool carry_sup);
   TrussOutcome truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k);
 
-  // state of the vertex peel (device)
-  uint32_t* lvl() { return lvl_.as<uint32_t>(); }
   uint32_t* deg() { return deg_.as<uint32_t>(); }
   uint64_t* off() { return off_.as<uint64_t>(); }
   uint32_t* adj() { return adj_.as<uint32_t>(); }
@@ -74,31 +108,42 @@ class Solver {
  private:
   void build_from_pairs(const uint32_t* d_pairs, uint64_t m, uint32_t n);
   void build_from_lines(const uint32_t* d_xs, const uint32_t* d_ys, uint64_t L, uint32_t n);
+  bool build_partitioned(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint32_t n);
   void solve_csr(kt_result* res);
   uint32_t read_u32(const uint32_t* dptr);
+  uint64_t read_u64(const uint64_t* dptr);
   void zero_u32(uint32_t* dptr);
   void exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n_plus_1);
   void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n_plus_1);
   uint32_t reduce_max_u32(const uint32_t* in, uint32_t n);
+  void sort_pairs_u64(const uint64_t* k_in, uint64_t* k_out, const uint32_t* v_in, uint32_t* v_out,
+                      uint32_t count);
+  LevelGraph& level0();
 
   int device_;
   hipStream_t stream_ = nullptr;
   Ctl* ctl_ = nullptr;
   uint32_t* pinned_ = nullptr;
-  hipEvent_t ev_[8];
+  hipEvent_t ev_[10];
   int lanes_count_ = 0, lanes_peel_ = 0;
   bool force_edge_count_ = false;
+  bool force_atomic_build_ = false;
 
   uint32_t n_ = 0;
   uint64_t m_ = 0;
   // full graph
-  DevBuf off_, adj_, deg_, lvl_, q0_, q1_, deg_bak_, cursor_;
+  DevBuf off_, adj_, deg_, lvl_, q0_, q1_, deg_bak_, cursor_, hist_, hoff_, part_;
+  std::vector<std::unique_ptr<LevelGraph>> levels_;
+  size_t depth_ = 0;  // levels in use
+  // scratch for level cuts and extraction
+  DevBuf list_, new_id_, keys_, keys2_, vals_, vals2_, to_;
   // core
-  DevBuf keys_, keys2_, to_, inv_, dout_, off_d_, adj_d_, src_, din_, din_off_, tkeys_, tvals_,
-      tiota_, in_src_, off_u_, adj_u_, eid_u_, sup_, state_, eq0_, eq1_, out_pairs_;
+  DevBuf inv_, inv_orig_, dout_, off_d_, adj_d_, src_, din_, din_off_, tkeys_, tvals_, tiota_,
+      in_src_, off_u_, adj_u_, eid_u_, sup_, state_, eq0_, eq1_, out_pairs_, del_thr_, orig0_;
+  EdgeSet work_[2];
   DevBuf temp_;
   CoreView last_core_{};
-  uint32_t last_floor_ = 0;
+  int32_t last_floor_ = -1;
   uint64_t last_truss_ = 0;
   bool have_truss_ = false;
 };

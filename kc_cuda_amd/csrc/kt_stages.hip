@@ -1,0 +1,510 @@
+// Stage logic of the device-resident two-pass k-max-truss solve (host side).
+//
+// Flow (reference: src/kc_gpu.cpp:129-208, 336-341; src/kc.cu:770-884, 1140-1309):
+//   (b) vertex peel with a galloping threshold -> max-core C0. When a step
+//       leaves few vertices alive, the rest of the peel continues on their
+//       induced subgraph (a new level with dense ids), so high thresholds
+//       never rescan the rows of vertices already gone.
+//   pass 1: rank + orient C0, exact supports, truss levels -> k̂ <= kmax.
+//   pass 2: the (L-1)-core C, L = max(k̂, 3) (it holds the kmax-truss),
+//       recovered from the deepest level that contains it; rank + orient,
+//       exact supports, truss levels upward from L with the reference's early
+//       exit (first level equal to pass 1's truss, src/kc.cu:1240-1241).
+//   Peeling drops a frontier holding a third of the live edges and recounts
+//   the rest (src/kc.cu:988, 1206-1233), and compacts once half the edges
+//   are dead, carrying supports (src/kc.cu:1174-1178).
+// Only scalar counters cross to the host.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+#include "kt_kernels.h"
+#include "kt_solver.h"
+
+namespace kt {
+
+namespace {
+inline int auto_lanes(double avg) {
+  if (avg >= 48) return 64;
+  if (avg >= 20) return 32;
+  if (avg >= 10) return 16;
+  return 8;
+}
+constexpr uint32_t kMinCut = 256;  // smaller alive sets stay on their level
+}  // namespace
+
+// ------------------------------------------------------ (b) vertex peel ---
+
+// Removes every alive vertex of degree < k, round by round; returns the
+// number removed. lvl[] of each removed vertex becomes k.
+uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
+  uint32_t* qc = ctl_->qcount;
+  uint32_t* q[2] = {q0_.as<uint32_t>(), q1_.as<uint32_t>()};
+  zero_u32(&qc[0]);
+  core_frontier_init(g.lvl, g.deg, g.n, k, q[0], &qc[0], stream_);
+  uint32_t c = read_u32(&qc[0]);
+  uint64_t removed = 0;
+  int cur = 0;
+  while (c) {
+    core_stamp(q[cur], &qc[cur], c, k, g.lvl, stream_);
+    zero_u32(&qc[cur ^ 1]);
+    core_process(g.off, g.adj, q[cur], &qc[cur], c, k, g.lvl, g.deg, q[cur ^ 1], &qc[cur ^ 1], stream_);
+    removed += c;
+    cur ^= 1;
+    c = read_u32(&qc[cur]);
+  }
+  return removed;
+}
+
+// Induced subgraph of the alive vertices of `p` as the next level.
+LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
+  if (levels_.size() <= depth_) levels_.emplace_back(new LevelGraph);
+  LevelGraph& s = *levels_[depth_];
+  list_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
+  new_id_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
+  zero_u32(&ctl_->nalive);
+  alive_list(p.lvl, p.n, list_.as<uint32_t>(), new_id_.as<uint32_t>(), &ctl_->nalive, stream_);
+  const uint32_t na = read_u32(&ctl_->nalive);
+  if (na != alive) fail("vertex peel bookkeeping mismatch");
+  s.b_deg.ensure(sizeof(uint32_t) * (size_t(na) + 1));
+  s.b_orig.ensure(sizeof(uint32_t) * (size_t(na) + 1));
+  s.b_off.ensure(sizeof(uint64_t) * (size_t(na) + 1));
+  s.b_lvl.ensure(sizeof(uint32_t) * (size_t(na) + 1));
+  s.n = na;
+  s.deg = s.b_deg.as<uint32_t>();
+  s.to_orig = s.b_orig.as<uint32_t>();
+  s.off = s.b_off.as<uint64_t>();
+  s.lvl = s.b_lvl.as<uint32_t>();
+  s.k_enter = k_enter;
+  s.ok.assign(1, k_enter);
+  fill_u32(s.deg + na, 1, 0, stream_);
+  sub_degrees(list_.as<uint32_t>(), p.deg, p.to_orig, na, s.deg, s.to_orig, stream_);
+  exclusive_scan_u32_to_u64(s.deg, s.off, na + 1);
+  s.slots = read_u64(s.off + na);
+  s.b_adj.ensure(sizeof(uint32_t) * std::max<uint64_t>(s.slots, 1));
+  s.adj = s.b_adj.as<uint32_t>();
+  sub_fill(p.off, p.adj, p.lvl, list_.as<uint32_t>(), new_id_.as<uint32_t>(), na, s.off, s.adj, stream_);
+  fill_u32(s.lvl, na, kNone, stream_);
+  ++depth_;
+  return s;
+}
+
+// Galloping search for the largest k with a non-empty k-core. Leaves the
+// max-core alive on the deepest level and returns its k (0: no edges).
+uint32_t Solver::max_core() {
+  depth_ = 1;
+  LevelGraph* g = &level0();
+  deg_bak_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
+  q0_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
+  q1_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
+  degrees_from_offsets(g->off, g->n, g->deg, stream_);
+  fill_u32(g->lvl, g->n, kNone, stream_);
+  const uint64_t gone = peel_vertices(*g, 1);
+  if (gone >= g->n) return 0;
+  uint64_t alive = g->n - gone;
+  uint32_t k = 1, step = 16;
+  bool grow = true;
+  g->ok.assign(1, 1);
+  if (alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, 1, uint32_t(alive));
+  while (step > 0) {
+    const uint32_t kk = k + step;
+    KT_HIP(hipMemcpyAsync(deg_bak_.p, g->deg, sizeof(uint32_t) * g->n, hipMemcpyDeviceToDevice, stream_));
+    const uint64_t removed = peel_vertices(*g, kk);
+    if (removed < alive) {
+      alive -= removed;
+      k = kk;
+      g->ok.push_back(k);
+      step = grow ? step * 2 : step / 2;
+      if (alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, k, uint32_t(alive));
+    } else {
+      // overshoot: re-open what this step removed and restore degrees
+      core_reopen_above(g->lvl, g->n, k, stream_);
+      KT_HIP(hipMemcpyAsync(g->deg, deg_bak_.p, sizeof(uint32_t) * g->n, hipMemcpyDeviceToDevice, stream_));
+      if (step == 1) break;
+      step /= 2;
+      grow = false;
+    }
+  }
+  return k;
+}
+
+// Leaves the k-core alive on the deepest level whose vertices all have core
+// number >= its entry threshold <= k: re-open everything that level removed
+// above its largest successful threshold kj <= k, recount degrees, peel at k.
+void Solver::core_at(uint32_t k) {
+  size_t d = depth_ - 1;
+  while (d > 0 && levels_[d]->k_enter > k) --d;
+  depth_ = d + 1;
+  LevelGraph& g = *levels_[d];
+  uint32_t kj = std::max<uint32_t>(g.k_enter, 1);
+  for (uint32_t t : g.ok)
+    if (t <= k) kj = std::max(kj, t);
+  core_reopen_above(g.lvl, g.n, kj, stream_);
+  core_alive_degrees(g.off, g.adj, g.lvl, g.n, g.deg, stream_);
+  if (k > kj) peel_vertices(g, k);
+}
+
+// --------------------------------------------------- (a1-a3) extraction ---
+
+void Solver::extract_core(LevelGraph& g, CoreView* cv) {
+  const uint32_t n = g.n;
+  const size_t nn = size_t(n_) + 1;
+  keys_.ensure(sizeof(uint64_t) * nn);
+  keys2_.ensure(sizeof(uint64_t) * nn);
+  vals_.ensure(sizeof(uint32_t) * nn);
+  vals2_.ensure(sizeof(uint32_t) * nn);
+  to_.ensure(sizeof(uint32_t) * nn);
+  zero_u32(&ctl_->nalive);
+  core_collect(g.lvl, g.deg, g.to_orig, n, keys_.as<uint64_t>(), vals_.as<uint32_t>(), &ctl_->nalive, stream_);
+  const uint32_t nc = read_u32(&ctl_->nalive);
+  *cv = CoreView{};
+  cv->nc = nc;
+  inv_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  inv_orig_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  dout_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  off_d_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  sort_pairs_u64(keys_.as<uint64_t>(), keys2_.as<uint64_t>(), vals_.as<uint32_t>(), vals2_.as<uint32_t>(), nc);
+  fill_u32(to_.as<uint32_t>(), n, kNone, stream_);
+  core_assign_rank(keys2_.as<uint64_t>(), vals2_.as<uint32_t>(), nc, to_.as<uint32_t>(), inv_.as<uint32_t>(),
+                   inv_orig_.as<uint32_t>(), stream_);
+  fill_u32(dout_.as<uint32_t>(), uint64_t(nc) + 1, 0, stream_);
+  orient_count(g.off, g.adj, to_.as<uint32_t>(), inv_.as<uint32_t>(), nc, dout_.as<uint32_t>(), stream_);
+  exclusive_scan_u32(dout_.as<uint32_t>(), off_d_.as<uint32_t>(), nc + 1);
+  const uint32_t mc = read_u32(off_d_.as<uint32_t>() + nc);
+  if (mc >= 0x7FFFFFFFu) fail("core subgraph exceeds 2^31 edges");
+  cv->mc = mc;
+  cv->max_dout = reduce_max_u32(dout_.as<uint32_t>(), nc);
+  const size_t mm = std::max<uint32_t>(mc, 1);
+  adj_d_.ensure(sizeof(uint32_t) * mm);
+  src_.ensure(sizeof(uint32_t) * mm);
+  orient_fill(g.off, g.adj, to_.as<uint32_t>(), inv_.as<uint32_t>(), nc, off_d_.as<uint32_t>(),
+              adj_d_.as<uint32_t>(), src_.as<uint32_t>(), nullptr, stream_);
+  // rows ascending
+  if (cv->max_dout <= 8192) {
+    sort_rows(off_d_.as<uint32_t>(), nc, adj_d_.as<uint32_t>(), cv->max_dout, stream_);
+  } else {
+    tkeys_.ensure(sizeof(uint32_t) * mm);
+    size_t tb = 0;
+    KT_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, adj_d_.as<uint32_t>(), tkeys_.as<uint32_t>(),
+                                                      mc, nc, off_d_.as<uint32_t>(), off_d_.as<uint32_t>() + 1,
+                                                      0, 32, stream_));
+    temp_.ensure(tb);
+    KT_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(temp_.p, tb, adj_d_.as<uint32_t>(), tkeys_.as<uint32_t>(),
+                                                      mc, nc, off_d_.as<uint32_t>(), off_d_.as<uint32_t>() + 1,
+                                                      0, 32, stream_));
+    KT_HIP(hipMemcpyAsync(adj_d_.p, tkeys_.p, sizeof(uint32_t) * mc, hipMemcpyDeviceToDevice, stream_));
+  }
+  orig0_.ensure(sizeof(uint32_t) * mm);
+  del_thr_.ensure(sizeof(uint32_t) * mm);
+  sup_.ensure(sizeof(int32_t) * mm);
+  state_.ensure(sizeof(uint32_t) * mm);
+  eq0_.ensure(sizeof(uint32_t) * mm);
+  eq1_.ensure(sizeof(uint32_t) * mm);
+  iota_u32(orig0_.as<uint32_t>(), mc, stream_);
+  fill_u32(del_thr_.as<uint32_t>(), mc, kNone, stream_);
+  fill_u32(state_.as<uint32_t>(), mc, kNone, stream_);
+  cv->inv = inv_orig_.as<uint32_t>();
+  cv->off_d = cv->off_d0 = off_d_.as<uint32_t>();
+  cv->adj_d = cv->adj_d0 = adj_d_.as<uint32_t>();
+  cv->src = cv->src0 = src_.as<uint32_t>();
+  cv->orig = orig0_.as<uint32_t>();
+  cv->del_thr = del_thr_.as<uint32_t>();
+  cv->mc0 = mc;
+  cv->max_dout0 = cv->max_dout;
+  cv->sup = sup_.as<int32_t>();
+  cv->state = state_.as<uint32_t>();
+  cv->work = -1;
+  build_undirected_view(cv);
+}
+
+// Undirected CSR of the current oriented edges: row w = [sources of w's
+// in-edges, ascending] ++ out(w), with the oriented edge id of every slot.
+void Solver::build_undirected_view(CoreView* cv) {
+  const uint32_t nc = cv->nc, mc = cv->mc;
+  const size_t mm = std::max<uint32_t>(mc, 1);
+  din_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  din_off_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  off_u_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  tkeys_.ensure(sizeof(uint32_t) * mm);
+  tvals_.ensure(sizeof(uint32_t) * mm);
+  tiota_.ensure(sizeof(uint32_t) * mm);
+  in_src_.ensure(sizeof(uint32_t) * mm);
+  adj_u_.ensure(sizeof(uint32_t) * 2 * mm);
+  eid_u_.ensure(sizeof(uint32_t) * 2 * mm);
+  fill_u32(din_.as<uint32_t>(), uint64_t(nc) + 1, 0, stream_);
+  count_in_degrees(cv->adj_d, mc, din_.as<uint32_t>(), stream_);
+  iota_u32(tiota_.as<uint32_t>(), mc, stream_);
+  if (mc) {
+    int bits = 1;
+    while (bits < 32 && (uint64_t(1) << bits) < nc) ++bits;
+    size_t tb = 0;
+    KT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, cv->adj_d, tkeys_.as<uint32_t>(), tiota_.as<uint32_t>(),
+                                              tvals_.as<uint32_t>(), mc, 0, bits, stream_));
+    temp_.ensure(tb);
+    KT_HIP(hipcub::DeviceRadixSort::SortPairs(temp_.p, tb, cv->adj_d, tkeys_.as<uint32_t>(), tiota_.as<uint32_t>(),
+                                              tvals_.as<uint32_t>(), mc, 0, bits, stream_));
+  }
+  gather_u32(tvals_.as<uint32_t>(), cv->src, mc, in_src_.as<uint32_t>(), stream_);
+  exclusive_scan_u32(din_.as<uint32_t>(), din_off_.as<uint32_t>(), nc + 1);
+  add_offsets(din_off_.as<uint32_t>(), cv->off_d, nc + 1, off_u_.as<uint32_t>(), stream_);
+  build_undirected(cv->off_d, cv->adj_d, din_off_.as<uint32_t>(), in_src_.as<uint32_t>(), tvals_.as<uint32_t>(),
+                   nc, off_u_.as<uint32_t>(), adj_u_.as<uint32_t>(), eid_u_.as<uint32_t>(), stream_);
+  cv->off_u = off_u_.as<uint32_t>();
+  cv->adj_u = adj_u_.as<uint32_t>();
+  cv->eid_u = eid_u_.as<uint32_t>();
+}
+
+// ------------------------------------------------------------ (c) count ---
+
+void Solver::count_support(CoreView* cv, float* ms) {
+  fill_u32(reinterpret_cast<uint32_t*>(cv->sup), cv->mc, 0, stream_);
+  const double avg = cv->nc ? double(cv->mc) / cv->nc : 0.0;
+  const int lanes = lanes_count_ ? lanes_count_ : auto_lanes(avg);
+  KT_HIP(hipEventRecord(ev_[8], stream_));
+  if (!force_edge_count_ && support_bitmap_fits(cv->nc, cv->max_dout))
+    support_bitmap(cv->off_d, cv->adj_d, cv->nc, cv->max_dout, lanes, cv->sup, ctl_, stream_);
+  else
+    support_edges(cv->off_d, cv->adj_d, cv->src, cv->nc, cv->mc, lanes, cv->sup, stream_);
+  KT_HIP(hipEventRecord(ev_[9], stream_));
+  if (ms) {
+    KT_HIP(hipEventSynchronize(ev_[9]));
+    KT_HIP(hipEventElapsedTime(ms, ev_[8], ev_[9]));
+  }
+}
+
+// ----------------------------------------------------- (d3) compaction ---
+
+void Solver::compact_core(CoreView* cv, bool carry_sup) {
+  const int t = cv->work == 0 ? 1 : 0;
+  EdgeSet& w = work_[t];
+  const uint32_t nc = cv->nc;
+  dout_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  fill_u32(dout_.as<uint32_t>(), uint64_t(nc) + 1, 0, stream_);
+  row_alive_count(cv->off_d, cv->state, nc, dout_.as<uint32_t>(), stream_);
+  w.off.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  exclusive_scan_u32(dout_.as<uint32_t>(), w.off.as<uint32_t>(), nc + 1);
+  const uint32_t mc2 = read_u32(w.off.as<uint32_t>() + nc);
+  const uint32_t max_dout = reduce_max_u32(dout_.as<uint32_t>(), nc);
+  const size_t mm = std::max<uint32_t>(mc2, 1);
+  w.adj.ensure(sizeof(uint32_t) * mm);
+  w.src.ensure(sizeof(uint32_t) * mm);
+  w.orig.ensure(sizeof(uint32_t) * mm);
+  w.sup.ensure(sizeof(int32_t) * mm);
+  row_compact(cv->off_d, cv->adj_d, cv->orig, cv->sup, cv->state, nc, w.off.as<uint32_t>(),
+              w.adj.as<uint32_t>(), w.src.as<uint32_t>(), w.orig.as<uint32_t>(),
+              carry_sup ? w.sup.as<int32_t>() : nullptr, stream_);
+  cv->off_d = w.off.as<uint32_t>();
+  cv->adj_d = w.adj.as<uint32_t>();
+  cv->src = w.src.as<uint32_t>();
+  cv->orig = w.orig.as<uint32_t>();
+  cv->sup = w.sup.as<int32_t>();
+  cv->mc = mc2;
+  cv->max_dout = max_dout;
+  cv->work = t;
+  build_undirected_view(cv);
+  fill_u32(cv->state, mc2, kNone, stream_);
+  if (!carry_sup) count_support(cv, nullptr);
+}
+
+// ------------------------------------------------------------- (d) peel ---
+
+// Truss levels on the core view, upward from threshold thr0 (an edge stays
+// while its support >= thr, i.e. in the (thr+2)-truss). Empty levels are
+// skipped by jumping to the minimum live support. With early_ans != 0, a
+// first level of exactly early_ans edges at kmax early_k ends the search
+// (src/kc.cu:1240-1241).
+TrussOutcome Solver::truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k) {
+  TrussOutcome out;
+  if (cv->mc == 0) return out;
+  uint32_t* qc = ctl_->qcount;
+  uint32_t* q[2] = {eq0_.as<uint32_t>(), eq1_.as<uint32_t>()};
+  uint64_t alive = cv->mc;
+  uint32_t r = 1;
+  int32_t thr = thr0;
+  bool first = true;
+  int cur = 0;
+  auto frontier = [&]() -> uint32_t {
+    cur = 0;
+    zero_u32(&qc[0]);
+    edge_frontier_init(cv->state, cv->sup, cv->mc, thr, q[0], &qc[0], stream_);
+    return read_u32(&qc[0]);
+  };
+  for (;;) {
+    if (2 * alive <= cv->mc) {  // half the edges are dead: compact, keep supports
+      compact_core(cv, true);
+      ++out.compactions;
+      r = 1;
+    }
+    uint32_t c = frontier();
+    while (c) {
+      edge_stamp_thr(q[cur], &qc[cur], c, r, uint32_t(thr), cv->state, cv->orig, cv->del_thr, stream_);
+      ++out.rounds;
+      if (c == alive) {  // every live edge goes: the level is empty
+        ++r;
+        alive = 0;
+        break;
+      }
+      if (3ull * c >= alive) {  // drop the frontier and recount the rest exactly
+        compact_core(cv, false);
+        ++out.compactions;
+        alive = cv->mc;
+        r = 1;
+        c = frontier();
+        continue;
+      }
+      const double avg = cv->nc ? double(2ull * cv->mc) / cv->nc : 0.0;
+      const int lanes = lanes_peel_ ? lanes_peel_ : auto_lanes(avg);
+      zero_u32(&qc[cur ^ 1]);
+      peel_process(cv->off_u, cv->adj_u, cv->eid_u, cv->src, cv->adj_d, q[cur], &qc[cur], c, r, thr, lanes,
+                   cv->state, cv->sup, q[cur ^ 1], &qc[cur ^ 1], stream_);
+      alive -= c;
+      ++r;
+      cur ^= 1;
+      c = read_u32(&qc[cur]);
+      if (c && 2 * alive <= cv->mc) {
+        compact_core(cv, true);
+        ++out.compactions;
+        if (cv->mc != alive) fail("edge peel bookkeeping mismatch");
+        r = 1;
+        c = frontier();
+      }
+    }
+    if (alive == 0) break;
+    fill_u32(&ctl_->minsup, 1, kNone, stream_);
+    edge_min_support(cv->state, cv->sup, cv->mc, &ctl_->minsup, stream_);
+    const uint32_t ms = read_u32(&ctl_->minsup);
+    out.kmax = ms + 2;
+    out.count = alive;
+    out.floor_thr = thr;
+    ++out.levels;
+    if (first && early_ans && alive == early_ans && out.kmax == early_k) break;
+    first = false;
+    thr = int32_t(ms) + 1;
+  }
+  return out;
+}
+
+// ------------------------------------------------------------ full solve ---
+
+void Solver::solve_pairs_device(const uint32_t* d_pairs, uint64_t m, uint32_t n, kt_result* res) {
+  KT_HIP(hipSetDevice(device_));
+  std::memset(res, 0, sizeof(*res));
+  KT_HIP(hipEventRecord(ev_[0], stream_));
+  build_from_pairs(d_pairs, m, n);
+  solve_csr(res);
+}
+
+void Solver::solve_lines_device(const uint32_t* d_xs, const uint32_t* d_ys, uint64_t L, uint32_t n,
+                                kt_result* res) {
+  KT_HIP(hipSetDevice(device_));
+  std::memset(res, 0, sizeof(*res));
+  KT_HIP(hipEventRecord(ev_[0], stream_));
+  build_from_lines(d_xs, d_ys, L, n);
+  solve_csr(res);
+}
+
+void Solver::solve_csr(kt_result* res) {
+  res->n = n_;
+  res->m = m_;
+  have_truss_ = false;
+  last_core_ = CoreView{};
+  last_truss_ = 0;
+  last_floor_ = -1;
+  KT_HIP(hipEventRecord(ev_[1], stream_));
+
+  res->max_core = max_core();
+  KT_HIP(hipEventRecord(ev_[2], stream_));
+  if (res->max_core == 0) {  // no edges
+    KT_HIP(hipEventSynchronize(ev_[2]));
+    KT_HIP(hipEventElapsedTime(&res->ms_total, ev_[0], ev_[2]));
+    KT_HIP(hipEventElapsedTime(&res->ms_build, ev_[0], ev_[1]));
+    KT_HIP(hipEventElapsedTime(&res->ms_kcore, ev_[1], ev_[2]));
+    have_truss_ = true;
+    return;
+  }
+
+  // pass 1 on the max-core
+  CoreView cv;
+  extract_core(*levels_[depth_ - 1], &cv);
+  count_support(&cv, nullptr);
+  TrussOutcome p1 = truss_search(&cv, 1, 0, 0);
+  res->k_hat = p1.kmax;
+  res->k_hat_edges = p1.count;
+  KT_HIP(hipEventRecord(ev_[3], stream_));
+
+  // pass 2 on the (L-1)-core
+  const uint32_t L = std::max<uint32_t>(p1.kmax, 3);
+  res->core_k = L - 1;
+  core_at(L - 1);
+  KT_HIP(hipEventRecord(ev_[4], stream_));
+  extract_core(*levels_[depth_ - 1], &cv);
+  res->n_core = cv.nc;
+  res->m_core = cv.mc;
+  KT_HIP(hipEventRecord(ev_[5], stream_));
+  count_support(&cv, &res->ms_count_kernel);
+  res->lanes = lanes_count_ ? lanes_count_ : auto_lanes(cv.nc ? double(cv.mc) / cv.nc : 0.0);
+  KT_HIP(hipEventRecord(ev_[6], stream_));
+  TrussOutcome p2 = truss_search(&cv, int32_t(L) - 2, p1.kmax >= 3 ? p1.count : 0, p1.kmax);
+  KT_HIP(hipEventRecord(ev_[7], stream_));
+  KT_HIP(hipEventSynchronize(ev_[7]));
+  res->kmax = p2.kmax;
+  res->truss_edges = p2.kmax ? p2.count : 0;
+  res->peel_rounds = p1.rounds + p2.rounds;
+  res->levels = p1.levels + p2.levels;
+  float t_a = 0, t_b = 0;
+  KT_HIP(hipEventElapsedTime(&res->ms_total, ev_[0], ev_[7]));
+  KT_HIP(hipEventElapsedTime(&res->ms_build, ev_[0], ev_[1]));
+  KT_HIP(hipEventElapsedTime(&t_a, ev_[1], ev_[2]));
+  KT_HIP(hipEventElapsedTime(&res->ms_pass1, ev_[2], ev_[3]));
+  KT_HIP(hipEventElapsedTime(&t_b, ev_[3], ev_[4]));
+  res->ms_kcore = t_a + t_b;
+  KT_HIP(hipEventElapsedTime(&res->ms_extract, ev_[4], ev_[5]));
+  KT_HIP(hipEventElapsedTime(&res->ms_count, ev_[5], ev_[6]));
+  KT_HIP(hipEventElapsedTime(&res->ms_peel, ev_[6], ev_[7]));
+  last_core_ = cv;
+  last_floor_ = p2.floor_thr;
+  last_truss_ = res->truss_edges;
+  have_truss_ = true;
+}
+
+void Solver::last_stats(kt_stats* out) {
+  if (!have_truss_) fail("no solve has completed on this context");
+  std::memset(out, 0, sizeof(*out));
+  const CoreView& cv = last_core_;
+  out->n_core = cv.nc;
+  out->m_core = cv.mc0;
+  out->truss_edges = last_truss_;
+  if (cv.mc0 == 0) return;
+  out->max_out_degree = cv.max_dout0;
+  DevBuf tmp;
+  tmp.ensure(16);
+  fill_u64(tmp.as<uint64_t>(), 2, 0, stream_);
+  const int32_t floor = last_truss_ ? last_floor_ : INT_MAX;
+  w_stats(cv.off_d0, cv.src0, cv.adj_d0, cv.del_thr, cv.mc0, floor, tmp.as<uint64_t>(), stream_);
+  uint64_t h[2];
+  KT_HIP(hipMemcpyAsync(h, tmp.p, 16, hipMemcpyDeviceToHost, stream_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  tmp.release();
+  out->w_core = h[0];
+  out->w_peeled = h[1];
+}
+
+uint64_t Solver::truss_edges(uint32_t* host_pairs, uint64_t cap) {
+  if (!have_truss_) fail("no solve has completed on this context");
+  const CoreView& cv = last_core_;
+  if (cv.mc0 == 0 || last_truss_ == 0) return 0;
+  out_pairs_.ensure(sizeof(uint32_t) * 2 * size_t(cv.mc0));
+  zero_u32(&ctl_->nalive);
+  truss_pairs(cv.del_thr, cv.mc0, last_floor_, cv.src0, cv.adj_d0, cv.inv, out_pairs_.as<uint32_t>(),
+              &ctl_->nalive, stream_);
+  const uint32_t cnt = read_u32(&ctl_->nalive);
+  const uint64_t take = std::min<uint64_t>(cnt, cap);
+  if (take && host_pairs)
+    KT_HIP(hipMemcpyAsync(host_pairs, out_pairs_.p, sizeof(uint32_t) * 2 * take, hipMemcpyDeviceToHost, stream_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  return cnt;
+}
+
+}  // namespace kt

@@ -31,7 +31,7 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 from kc_cuda_amd import gen  # noqa: E402
 from oracle import oracle as O  # noqa: E402
-from graphs import closed_forms, random_graph  # noqa: E402
+from graphs import closed_forms, make_workload, random_graph  # noqa: E402
 
 SMALL = {
     "er16k_clique24": dict(kind="er_clique", n=16384, avg_deg=16.0, clique=24, seed=0),
@@ -48,16 +48,7 @@ LARGE = {
 }
 
 
-def make(params: dict):
-    p = dict(params)
-    kind = p.pop("kind")
-    if kind == "er_clique":
-        return gen.er_clique(**p)
-    if kind == "rmat":
-        return gen.rmat(**p)
-    if kind == "chung_lu":
-        return gen.chung_lu(**p)
-    raise KeyError(kind)
+make = make_workload
 
 
 def kc_cpu_case(name: str, params: dict, tmp: Path) -> dict:

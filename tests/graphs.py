@@ -84,3 +84,17 @@ def brute_kmax_truss(pairs: np.ndarray, n: int):
         best_k, best = k, sorted(cur)
         k += 1
     return best_k, best
+
+
+def make_workload(params: dict):
+    """Generate a graph from a golden-fixture parameter record."""
+    from kc_cuda_amd import gen
+    p = dict(params)
+    kind = p.pop("kind")
+    if kind == "er_clique":
+        return gen.er_clique(**p)
+    if kind == "rmat":
+        return gen.rmat(**p)
+    if kind == "chung_lu":
+        return gen.chung_lu(**p)
+    raise KeyError(kind)

@@ -86,12 +86,23 @@ def test_rmat_vs_golden(ctx, scale):
         np.testing.assert_array_equal(ctx.truss_edges(), canon_pairs(ref.truss))
 
 
-def test_csr_matches_oracle(ctx):
+@pytest.mark.parametrize("atomic_build", [False, True])
+@pytest.mark.parametrize("lines", [False, True])
+def test_csr_matches_oracle(ctx, atomic_build, lines):
     from kc_cuda_amd import gen
     O = _oracle()
     pairs, n = gen.rmat(12, 8, 0.57, 0.19, 0.19, True, 0, 5)
-    ctx.solve_pairs(pairs, n)
-    off, adj = ctx.last_csr(n, 2 * len(pairs))
+    print(f"[gpu] csr atomic_build={atomic_build} lines={lines}", flush=True)
+    try:
+        ctx.tune(force_atomic_build=atomic_build)
+        if lines:
+            xs, ys = O.lines_from_pairs(pairs)
+            ctx.solve_lines(xs, ys, n)
+        else:
+            ctx.solve_pairs(pairs, n)
+        off, adj = ctx.last_csr(n, 2 * len(pairs))
+    finally:
+        ctx.tune()
     ooff, oadj, _ = O.csr_from_pairs(pairs, n)
     np.testing.assert_array_equal(off, ooff)
     for v in range(n):  # rows are sets: the device scatter order is free
