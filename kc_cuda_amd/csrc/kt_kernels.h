@@ -108,6 +108,10 @@ void edges_alive_above(const uint32_t* state, uint32_t mc, uint32_t stamp_floor,
 void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, const uint32_t* del_thr,
              uint32_t mc, int32_t floor, uint64_t* out2, hipStream_t s);
 
+void row_src(const uint32_t* off, uint32_t rows, uint32_t* src, hipStream_t s);
+void sons(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, uint32_t nc,
+          uint32_t* lson, uint32_t* rson, hipStream_t s);
+
 // (d3) compaction of the live edges and deletion bookkeeping on base ids
 void row_alive_count(const uint32_t* off_d, const uint32_t* state, uint32_t nc, uint32_t* cnt,
                      hipStream_t s);

@@ -740,6 +740,33 @@ __global__ void k_alive_pairs(const uint32_t* __restrict__ state, uint32_t mc, u
   }
 }
 
+// Row index of every slot of a CSR with uint32 offsets.
+__global__ __launch_bounds__(kBlock) void k_row_src(const uint32_t* __restrict__ off, uint32_t rows,
+                                                    uint32_t* src) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t r = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < rows; r += waves)
+    for (uint32_t j = off[r] + lane; j < off[r + 1]; j += kWave) src[j] = r;
+}
+
+// For every oriented edge e = u -> v (u < v), the slot of e in row u (lson)
+// and in row v (rson) of the undirected view (the reference's left_son /
+// right_son, src/kc.cu:1656-1666).
+__global__ __launch_bounds__(kBlock) void k_sons(const uint32_t* __restrict__ off_u,
+                                                 const uint32_t* __restrict__ adj_u,
+                                                 const uint32_t* __restrict__ eid_u, uint32_t nc,
+                                                 uint32_t* lson, uint32_t* rson) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t w = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; w < nc; w += waves)
+    for (uint32_t j = off_u[w] + lane; j < off_u[w + 1]; j += kWave) {
+      if (w < adj_u[j])
+        lson[eid_u[j]] = j;
+      else
+        rson[eid_u[j]] = j;
+    }
+}
+
 // ---------------------------------------------------- (d3) compaction ---
 
 __global__ __launch_bounds__(kBlock) void k_row_alive_count(const uint32_t* __restrict__ off_d,
@@ -853,6 +880,13 @@ void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, co
   if (mc)
     k_w_stats<<<grid_for(mc), kBlock, 0, s>>>(off_d, src, dst, del_thr, mc, floor,
                                               reinterpret_cast<unsigned long long*>(out2));
+}
+void row_src(const uint32_t* off, uint32_t rows, uint32_t* src, hipStream_t s) {
+  if (rows) k_row_src<<<grid_for(uint64_t(rows) * kWave), kBlock, 0, s>>>(off, rows, src);
+}
+void sons(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, uint32_t nc, uint32_t* lson,
+          uint32_t* rson, hipStream_t s) {
+  if (nc) k_sons<<<grid_for(uint64_t(nc) * kWave), kBlock, 0, s>>>(off_u, adj_u, eid_u, nc, lson, rson);
 }
 void row_alive_count(const uint32_t* off_d, const uint32_t* state, uint32_t nc, uint32_t* cnt, hipStream_t s) {
   if (nc) k_row_alive_count<<<grid_for(uint64_t(nc) * kWave), kBlock, 0, s>>>(off_d, state, nc, cnt);

@@ -89,7 +89,10 @@ class Solver {
   // stages (kt_stages.hip)
   void load_csr_host(uint32_t n, const uint64_t* off, const uint32_t* adj);
   uint64_t peel_vertices(LevelGraph& g, uint32_t k);
-  uint32_t max_core();
+  // fresh: start from full degrees with every vertex alive; otherwise from
+  // the state already on level 0. allow_cut: continue on induced subgraphs.
+  uint32_t max_core(bool allow_cut = true, bool fresh = true);
+  LevelGraph& level0();
   LevelGraph& cut_level(LevelGraph& parent, uint32_t k_enter, uint32_t alive);
   void core_at(uint32_t k);  // leaves the k-core alive on the deepest usable level
   void extract_core(LevelGraph& g, CoreView* cv);
@@ -98,7 +101,23 @@ class Solver {
   // Keep only the live edges (state == kNone) of the current view; with
   // carry_sup their supports are kept, otherwise recounted exactly.
   void compact_core(CoreView* cv, bool carry_sup);
-  TrussOutcome truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k);
+  // exact_early: stop at a first level of early_ans edges and report early_k
+  // whatever the minimum support (the reference's rule, src/kc.cu:1240).
+  TrussOutcome truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k,
+                            bool exact_early = false);
+
+  // reference-compatible stage entry points (kt_compat.hip)
+  void vertex_state_from_host(const uint32_t* deg_origin, const uint32_t* mask);
+  void level0_mask_to_host(uint32_t* mask);
+  void level0_degrees_to_host(uint32_t* deg);
+  void orient_with_ranks(const uint32_t* h_to, uint32_t new_n, const uint32_t* h_off_d,
+                         uint32_t* h_adj_d);
+  void view_from_oriented_host(uint32_t nc, const uint32_t* h_off_d, const uint32_t* h_adj_d,
+                               CoreView* cv);
+  void undirected_to_host(const CoreView& cv, uint32_t* adj_u, uint32_t* eid_u, uint32_t* lson,
+                          uint32_t* rson);
+  void core_ranks_to_host(const CoreView& cv, uint32_t* inv_orig, uint32_t* dout);
+  void support_to_host(const CoreView& cv, int32_t* sup);
 
   uint32_t* deg() { return deg_.as<uint32_t>(); }
   uint64_t* off() { return off_.as<uint64_t>(); }
@@ -118,7 +137,6 @@ class Solver {
   uint32_t reduce_max_u32(const uint32_t* in, uint32_t n);
   void sort_pairs_u64(const uint64_t* k_in, uint64_t* k_out, const uint32_t* v_in, uint32_t* v_out,
                       uint32_t count);
-  LevelGraph& level0();
 
   int device_;
   hipStream_t stream_ = nullptr;

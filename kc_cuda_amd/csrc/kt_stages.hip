@@ -93,21 +93,28 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
 
 // Galloping search for the largest k with a non-empty k-core. Leaves the
 // max-core alive on the deepest level and returns its k (0: no edges).
-uint32_t Solver::max_core() {
+uint32_t Solver::max_core(bool allow_cut, bool fresh) {
   depth_ = 1;
-  LevelGraph* g = &level0();
+  LevelGraph* g = levels_.empty() || fresh ? &level0() : levels_[0].get();
   deg_bak_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
   q0_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
   q1_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
-  degrees_from_offsets(g->off, g->n, g->deg, stream_);
-  fill_u32(g->lvl, g->n, kNone, stream_);
+  uint64_t before = g->n;
+  if (fresh) {
+    degrees_from_offsets(g->off, g->n, g->deg, stream_);
+    fill_u32(g->lvl, g->n, kNone, stream_);
+  } else {
+    zero_u32(&ctl_->nalive);
+    alive_list(g->lvl, g->n, q1_.as<uint32_t>(), q0_.as<uint32_t>(), &ctl_->nalive, stream_);
+    before = read_u32(&ctl_->nalive);
+  }
   const uint64_t gone = peel_vertices(*g, 1);
-  if (gone >= g->n) return 0;
-  uint64_t alive = g->n - gone;
+  if (gone >= before) return 0;
+  uint64_t alive = before - gone;
   uint32_t k = 1, step = 16;
   bool grow = true;
   g->ok.assign(1, 1);
-  if (alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, 1, uint32_t(alive));
+  if (allow_cut && alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, 1, uint32_t(alive));
   while (step > 0) {
     const uint32_t kk = k + step;
     KT_HIP(hipMemcpyAsync(deg_bak_.p, g->deg, sizeof(uint32_t) * g->n, hipMemcpyDeviceToDevice, stream_));
@@ -117,7 +124,7 @@ uint32_t Solver::max_core() {
       k = kk;
       g->ok.push_back(k);
       step = grow ? step * 2 : step / 2;
-      if (alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, k, uint32_t(alive));
+      if (allow_cut && alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, k, uint32_t(alive));
     } else {
       // overshoot: re-open what this step removed and restore degrees
       core_reopen_above(g->lvl, g->n, k, stream_);
@@ -315,7 +322,8 @@ void Solver::compact_core(CoreView* cv, bool carry_sup) {
 // skipped by jumping to the minimum live support. With early_ans != 0, a
 // first level of exactly early_ans edges at kmax early_k ends the search
 // (src/kc.cu:1240-1241).
-TrussOutcome Solver::truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k) {
+TrussOutcome Solver::truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k,
+                                  bool exact_early) {
   TrussOutcome out;
   if (cv->mc == 0) return out;
   uint32_t* qc = ctl_->qcount;
@@ -379,7 +387,10 @@ TrussOutcome Solver::truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans
     out.count = alive;
     out.floor_thr = thr;
     ++out.levels;
-    if (first && early_ans && alive == early_ans && out.kmax == early_k) break;
+    if (first && early_ans && alive == early_ans && (exact_early || out.kmax == early_k)) {
+      out.kmax = early_k;
+      break;
+    }
     first = false;
     thr = int32_t(ms) + 1;
   }

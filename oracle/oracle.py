@@ -234,12 +234,17 @@ class RefRun:
         return self.t_all - self.t_read
 
 
-def run_kc_cpu(tsv: str | os.PathLike, threads: int | None = None, timeout: float = 600.0
-               ) -> RefRun:
-    """Run the reference kc_cpu on a TSV. Its parser sizes per-thread regions
-    by omp_get_num_procs() and indexes them by omp_get_thread_num()
+KC_GPU_LINKED = HERE / "_ref" / "kc_gpu_linked"
+
+
+def run_kc_cpu(tsv: str | os.PathLike, threads: int | None = None, timeout: float = 600.0,
+               binary: Path | None = None) -> RefRun:
+    """Run the reference kc_cpu (or another binary built from the reference's
+    host driver, e.g. kc_gpu_linked) on a TSV. Its parser sizes per-thread
+    regions by omp_get_num_procs() and indexes them by omp_get_thread_num()
     (src/kc_cpu.cpp:782-805), so the process is pinned to exactly `threads`
     CPUs with OMP_NUM_THREADS equal to that count."""
+    KC = binary or KC_CPU
     cpus = sorted(os.sched_getaffinity(0))
     if threads is None:
         threads = len(cpus)
@@ -248,7 +253,7 @@ def run_kc_cpu(tsv: str | os.PathLike, threads: int | None = None, timeout: floa
     env = dict(os.environ)
     env["OMP_NUM_THREADS"] = str(threads)
     t0 = time.perf_counter()
-    p = subprocess.run([str(KC_CPU), "-f", str(tsv)], env=env, stdout=subprocess.PIPE,
+    p = subprocess.run([str(KC), "-f", str(tsv)], env=env, stdout=subprocess.PIPE,
                        stderr=subprocess.PIPE, timeout=timeout,
                        preexec_fn=lambda: os.sched_setaffinity(0, use))
     wall = time.perf_counter() - t0
