@@ -105,9 +105,13 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
  * (0 = automatic; 8, 16, 32 or 64) and kernel-variant flags:
  * KT_FORCE_EDGE_COUNT selects the per-edge support kernel instead of the LDS
  * bitmap one, KT_FORCE_ATOMIC_BUILD the atomic CSR build instead of the
- * bucketed one. Every variant computes the same result. */
+ * bucketed one, KT_FORCE_SPARSE / KT_FORCE_DENSE the sorted-row or the
+ * bit-matrix core representation (default: by density). Every variant
+ * computes the same result. */
 #define KT_FORCE_EDGE_COUNT 1
 #define KT_FORCE_ATOMIC_BUILD 2
+#define KT_FORCE_SPARSE 4
+#define KT_FORCE_DENSE 8
 int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
 
 /* Byte-model terms of the last solve (one extra device pass, untimed). */

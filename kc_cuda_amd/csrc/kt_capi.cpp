@@ -200,6 +200,7 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
   ctx->solver->set_lanes(count_lanes, peel_lanes);
   ctx->solver->set_force_edge_count((flags & KT_FORCE_EDGE_COUNT) != 0);
   ctx->solver->set_force_atomic_build((flags & KT_FORCE_ATOMIC_BUILD) != 0);
+  ctx->solver->set_core_mode((flags & KT_FORCE_SPARSE) ? 1 : (flags & KT_FORCE_DENSE) ? 2 : 0);
   g_err.clear();
   return 0;
 }

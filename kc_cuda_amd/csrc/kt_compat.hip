@@ -75,7 +75,7 @@ void Solver::vertex_state_from_host(const uint32_t* deg_origin, const uint32_t* 
   }
   KT_HIP(hipMemcpyAsync(g.lvl, l.data(), sizeof(uint32_t) * g.n, hipMemcpyHostToDevice, stream_));
   if (any || !deg_origin) {
-    core_alive_degrees(g.off, g.adj, g.lvl, g.n, g.deg, stream_);
+    core_alive_degrees(g.off, g.adj, g.lvl, g.n, g.deg, heavy(), stream_);
   } else {
     KT_HIP(hipMemcpyAsync(g.deg, deg_origin, sizeof(uint32_t) * g.n, hipMemcpyHostToDevice, stream_));
   }
@@ -187,6 +187,7 @@ void Solver::view_from_oriented_host(uint32_t nc, const uint32_t* h_off_d, const
   cv->state = state_.as<uint32_t>();
   cv->work = -1;
   build_undirected_view(cv);
+  if (core_mode_ == 2 ? nc <= 65536 : (core_mode_ == 0 && dense_fits(nc, mc))) make_dense(cv);
 }
 
 void Solver::undirected_to_host(const CoreView& cv, uint32_t* adj_u, uint32_t* eid_u, uint32_t* lson,

@@ -42,9 +42,10 @@ void core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32
                 uint32_t* lvl, hipStream_t s);
 void core_process(const uint64_t* off, const uint32_t* adj, const uint32_t* q,
                   const uint32_t* qcount, uint32_t maxq, uint32_t k, uint32_t* lvl,
-                  uint32_t* deg, uint32_t* qnext, uint32_t* qnext_count, hipStream_t s);
+                  uint32_t* deg, uint32_t* qnext, uint32_t* qnext_count, HeavyList hl,
+                  hipStream_t s);
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl,
-                        uint32_t n, uint32_t* deg, hipStream_t s);
+                        uint32_t n, uint32_t* deg, HeavyList hl, hipStream_t s);
 // lvl[v] = (lvl[v] > k) ? kNone : (lvl[v] == kNone ? kNone : k) -- used to
 // re-open every vertex whose removal threshold exceeds k.
 void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s);
@@ -60,9 +61,11 @@ void alive_list(const uint32_t* lvl, uint32_t n, uint32_t* list, uint32_t* new_i
                 uint32_t* count, hipStream_t s);
 void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_orig, uint32_t na,
                  uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s);
+// cursor: cursor_cap zeroed 64-bit slots, at least the number of rows of the
+// cut longer than the heavy threshold (slots / 8192 + 1 suffices)
 void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, const uint32_t* list,
               const uint32_t* new_id, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj,
-              hipStream_t s);
+              HeavyList hl, uint64_t* cursor, uint32_t cursor_cap, hipStream_t s);
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
                   const uint32_t* inv, uint32_t nc, uint32_t* dout, hipStream_t s);
 void orient_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
@@ -107,6 +110,21 @@ void edges_alive_above(const uint32_t* state, uint32_t mc, uint32_t stamp_floor,
 
 void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, const uint32_t* del_thr,
              uint32_t mc, int32_t floor, uint64_t* out2, hipStream_t s);
+
+// Dense cores: neighbourhoods as bit-matrix rows of ws = ceil(nc / 64) words.
+uint32_t dense_words(uint32_t nc);
+bool dense_fits(uint32_t nc, uint32_t mc);
+void dense_rows(const uint32_t* off_u, const uint32_t* adj_u, uint32_t nc, uint32_t ws,
+                uint64_t* bits0, uint64_t* bits, uint32_t* pref0, hipStream_t s);
+void dense_support(const uint32_t* src, const uint32_t* dst, uint32_t mc, const uint64_t* bits,
+                   uint32_t ws, const uint32_t* state, int32_t* sup, hipStream_t s);
+void dense_peel(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r, int32_t thr,
+                const uint32_t* src, const uint32_t* dst, const uint64_t* bits0,
+                const uint32_t* pref0, const uint64_t* bits, uint32_t ws, const uint32_t* off_u,
+                const uint32_t* eid_u, const uint32_t* state, int32_t* sup, uint32_t* qnext,
+                uint32_t* qnext_count, hipStream_t s);
+void dense_clear(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, const uint32_t* src,
+                 const uint32_t* dst, uint64_t* bits, uint32_t ws, hipStream_t s);
 
 void row_src(const uint32_t* off, uint32_t rows, uint32_t* src, hipStream_t s);
 void sons(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, uint32_t nc,

@@ -39,6 +39,65 @@ __device__ __forceinline__ void wave_append(bool pred, T item, T* q, uint32_t* c
   if (pred) q[base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = item;
 }
 
+// Block-wide exclusive scan of one value per thread (blockDim == kBlock).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t s_w[kBlock / kWave];
+  const int lane = lane64(), wid = int(threadIdx.x) / kWave;
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
+  }
+  if (lane == kWave - 1) s_w[wid] = inc;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kBlock / kWave; ++k) {
+    const uint32_t w = s_w[k];
+    if (k < wid) before += w;
+    tot += w;
+  }
+  __syncthreads();
+  *total = tot;
+  return before + inc - v;
+}
+
+// Stream compaction of [0, n) by tiles of kBlock * kTileItems indices: one
+// counter atomic per tile instead of one per wave (a single queue counter hit
+// by every wave of a 4M-vertex sweep serialises the whole launch).
+constexpr int kTileItems = 16;
+template <typename Pred, typename Emit>
+__device__ __forceinline__ void tile_compact(uint64_t n, Pred pred, Emit emit, uint32_t* count) {
+  __shared__ uint32_t s_base;
+  const uint64_t tile = uint64_t(blockDim.x) * kTileItems;
+  for (uint64_t t0 = uint64_t(blockIdx.x) * tile; t0 < n; t0 += uint64_t(gridDim.x) * tile) {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) {
+      const uint64_t i = t0 + uint64_t(k) * blockDim.x + threadIdx.x;
+      if (i < n && pred(i)) bits |= 1u << k;
+    }
+    uint32_t total;
+    const uint32_t pre = block_exclusive_scan(uint32_t(__popc(bits)), &total);
+    if (threadIdx.x == 0 && total) s_base = atomicAdd(count, total);
+    __syncthreads();
+    uint32_t at = s_base + pre;
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k)
+      if ((bits >> k) & 1u) emit(at++, t0 + uint64_t(k) * blockDim.x + threadIdx.x);
+    __syncthreads();
+  }
+}
+
+inline int grid_tiles(uint64_t n) {
+  const uint64_t tile = uint64_t(kBlock) * kTileItems;
+  uint64_t g = (n + tile - 1) / tile;
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return int(g);
+}
+
 template <int G>
 __device__ __forceinline__ uint32_t group_sum(uint32_t v) {
 #pragma unroll
@@ -276,14 +335,12 @@ __global__ void k_deg_from_off(const uint64_t* off, uint32_t n, uint32_t* deg) {
 
 // ----------------------------------------------------- (b) vertex peel ---
 
-__global__ void k_core_frontier(const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ deg,
-                                uint32_t n, uint32_t k, uint32_t* q, uint32_t* qcount) {
-  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < n;
-       base += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t v = base + threadIdx.x;
-    const bool p = v < n && lvl[v] == kNone && deg[v] < k;
-    wave_append<uint32_t>(p, uint32_t(v), q, qcount);
-  }
+__global__ __launch_bounds__(kBlock) void k_core_frontier(const uint32_t* __restrict__ lvl,
+                                                          const uint32_t* __restrict__ deg, uint32_t n,
+                                                          uint32_t k, uint32_t* q, uint32_t* qcount) {
+  tile_compact(
+      n, [&](uint64_t v) { return lvl[v] == kNone && deg[v] < k; },
+      [&](uint32_t at, uint64_t v) { q[at] = uint32_t(v); }, qcount);
 }
 
 __global__ void k_core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t k, uint32_t* lvl) {
@@ -292,18 +349,98 @@ __global__ void k_core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t
     lvl[q[i]] = k;
 }
 
+// Rows longer than kHeavyRow slots are not walked by one wave: the light
+// kernels push them to a heavy list and a persistent kernel then spreads each
+// heavy row over all of its blocks (hub rows of power-law graphs would
+// otherwise serialise a launch behind one wave).
+constexpr uint64_t kHeavyRow = 8192;
+constexpr int kHeavyGrid = 512;
+
+__device__ __forceinline__ void push_heavy(uint32_t item, uint32_t* heavy, uint32_t* heavy_count) {
+  heavy[atomicAdd(heavy_count, 1u)] = item;
+}
+
+// Heavy rows are cut into slices of kSlice slots: slices[h] = slices before
+// heavy row h, slices[hc] = total (one block scans the short heavy list).
+// The row of heavy item h is row_of(h) = list ? list[heavy[h]] : heavy[h].
+constexpr uint64_t kSlice = 4096;
+__global__ __launch_bounds__(kBlock) void k_heavy_plan(const uint64_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ heavy,
+                                                       const uint32_t* heavy_count,
+                                                       const uint32_t* __restrict__ list, uint32_t* slices) {
+  const uint32_t hc = *heavy_count;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < hc; c0 += blockDim.x) {
+    const uint32_t h = c0 + threadIdx.x;
+    uint32_t ns = 0;
+    if (h < hc) {
+      const uint32_t v = list ? list[heavy[h]] : heavy[h];
+      ns = uint32_t((off[v + 1] - off[v] + kSlice - 1) / kSlice);
+    }
+    uint32_t total;
+    const uint32_t pre = block_exclusive_scan(ns, &total);
+    if (h < hc) slices[h] = carry + pre;
+    carry += total;
+  }
+  if (threadIdx.x == 0) slices[hc] = carry;
+}
+
+// slice t -> heavy row h with slices[h] <= t < slices[h + 1]
+__device__ __forceinline__ uint32_t slice_row(const uint32_t* slices, uint32_t hc, uint32_t t) {
+  uint32_t lo = 0, hi = hc;  // answer in [0, hc)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (slices[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void core_visit(uint32_t w, bool valid, uint32_t k, const uint32_t* __restrict__ lvl,
+                                           uint32_t* deg, uint32_t* qnext, uint32_t* qnext_count) {
+  bool crossed = false;
+  if (valid && lvl[w] == kNone) crossed = (atomicSub(&deg[w], 1u) == k);
+  wave_append<uint32_t>(crossed, w, qnext, qnext_count);
+}
+
+__global__ __launch_bounds__(kBlock) void k_core_process_heavy(
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj, const uint32_t* heavy,
+    const uint32_t* heavy_count, const uint32_t* __restrict__ slices, uint32_t k,
+    const uint32_t* __restrict__ lvl, uint32_t* deg, uint32_t* qnext, uint32_t* qnext_count) {
+  const uint32_t hc = *heavy_count;
+  if (hc == 0) return;
+  const uint32_t total = slices[hc];
+  const int lane = lane64();
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const uint32_t h = slice_row(slices, hc, t);
+    const uint32_t u = heavy[h];
+    const uint64_t b = off[u] + uint64_t(t - slices[h]) * kSlice;
+    const uint64_t e = min(off[u + 1], b + kSlice);
+    for (uint64_t j0 = b + (threadIdx.x & ~63u); j0 < e; j0 += blockDim.x) {
+      const uint64_t j = j0 + lane;
+      core_visit(j < e ? adj[j] : 0, j < e, k, lvl, deg, qnext, qnext_count);
+    }
+  }
+}
+
 // One wave per removed vertex: every still-alive neighbour loses one degree;
 // the decrement that takes it from k to k-1 queues it (exactly once).
 __global__ __launch_bounds__(kBlock) void k_core_process(
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj, const uint32_t* q,
     const uint32_t* qcount, uint32_t k, const uint32_t* __restrict__ lvl, uint32_t* deg,
-    uint32_t* qnext, uint32_t* qnext_count) {
+    uint32_t* qnext, uint32_t* qnext_count, uint32_t* heavy, uint32_t* heavy_count) {
   const uint32_t cnt = *qcount;
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   const int lane = lane64();
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < cnt; i += waves) {
     const uint32_t u = q[i];
     const uint64_t b = off[u], e = off[u + 1];
+    if (e - b > kHeavyRow) {
+      if (lane == 0) push_heavy(u, heavy, heavy_count);
+      continue;
+    }
     for (uint64_t j0 = b; j0 < e; j0 += kWave) {
       const uint64_t j = j0 + lane;
       bool crossed = false;
@@ -317,14 +454,44 @@ __global__ __launch_bounds__(kBlock) void k_core_process(
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_core_alive_deg_heavy(const uint64_t* __restrict__ off,
+                                                                 const uint32_t* __restrict__ adj,
+                                                                 const uint32_t* __restrict__ lvl,
+                                                                 const uint32_t* heavy,
+                                                                 const uint32_t* heavy_count,
+                                                                 const uint32_t* __restrict__ slices,
+                                                                 uint32_t* deg) {
+  const uint32_t hc = *heavy_count;
+  if (hc == 0) return;
+  const uint32_t total = slices[hc];
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const uint32_t h = slice_row(slices, hc, t);
+    const uint32_t v = heavy[h];
+    const uint64_t b = off[v] + uint64_t(t - slices[h]) * kSlice;
+    const uint64_t e = min(off[v + 1], b + kSlice);
+    uint32_t c = 0;
+    for (uint64_t j = b + threadIdx.x; j < e; j += blockDim.x) c += (lvl[adj[j]] == kNone);
+    c = wave_sum(c);
+    if (lane64() == 0 && c) atomicAdd(&deg[v], c);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_core_alive_deg(const uint64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ adj,
                                                            const uint32_t* __restrict__ lvl,
-                                                           uint32_t n, uint32_t* deg) {
+                                                           uint32_t n, uint32_t* deg, uint32_t* heavy,
+                                                           uint32_t* heavy_count) {
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   const int lane = lane64();
   for (uint32_t v = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; v < n; v += waves) {
     if (lvl[v] != kNone) continue;  // wave-uniform
+    if (off[v + 1] - off[v] > kHeavyRow) {
+      if (lane == 0) {
+        deg[v] = 0;
+        push_heavy(v, heavy, heavy_count);
+      }
+      continue;
+    }
     uint32_t c = 0;
     for (uint64_t j = off[v] + lane; j < off[v + 1]; j += kWave) c += (lvl[adj[j]] == kNone);
     c = wave_sum(c);
@@ -343,26 +510,18 @@ __global__ void k_core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k) {
 
 // key = degree-in-core << 32 | (~id): ascending order = cmp_link's
 // (degree ascending, id descending), src/kc_gpu.cpp:27-30.
-__global__ void k_core_collect(const uint32_t* __restrict__ lvl, const uint32_t* __restrict__ deg,
-                               const uint32_t* __restrict__ to_orig, uint32_t n, uint64_t* keys,
-                               uint32_t* vals, uint32_t* count) {
-  const int lane = lane64();
-  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < n;
-       base += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t v = base + threadIdx.x;
-    const bool p = v < n && lvl[v] == kNone;
-    const uint64_t mask = __ballot(p);
-    if (mask == 0) continue;  // wave-uniform
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t at = 0;
-    if (lane == leader) at = atomicAdd(count, uint32_t(__popcll(mask)));
-    at = __shfl(at, leader) + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-    if (p) {
-      const uint32_t id = to_orig ? to_orig[v] : uint32_t(v);
-      keys[at] = uint64_t(deg[v]) << 32 | uint64_t(0xFFFFFFFFu - id);
-      vals[at] = uint32_t(v);
-    }
-  }
+__global__ __launch_bounds__(kBlock) void k_core_collect(const uint32_t* __restrict__ lvl,
+                                                         const uint32_t* __restrict__ deg,
+                                                         const uint32_t* __restrict__ to_orig, uint32_t n,
+                                                         uint64_t* keys, uint32_t* vals, uint32_t* count) {
+  tile_compact(
+      n, [&](uint64_t v) { return lvl[v] == kNone; },
+      [&](uint32_t at, uint64_t v) {
+        const uint32_t id = to_orig ? to_orig[v] : uint32_t(v);
+        keys[at] = uint64_t(deg[v]) << 32 | uint64_t(0xFFFFFFFFu - id);
+        vals[at] = uint32_t(v);
+      },
+      count);
 }
 
 // sorted (key, local id) -> rank maps; inv_orig carries the input's ids
@@ -379,24 +538,15 @@ __global__ void k_assign_rank(const uint64_t* keys, const uint32_t* vals, uint32
 // ---------------------------------------------- (b') induced subgraph ---
 
 // Induced subgraph of the alive vertices: sub[i] = alive vertex, new_id[v] = i.
-__global__ void k_alive_list(const uint32_t* __restrict__ lvl, uint32_t n, uint32_t* list,
-                             uint32_t* new_id, uint32_t* count) {
-  const int lane = lane64();
-  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < n;
-       base += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t v = base + threadIdx.x;
-    const bool p = v < n && lvl[v] == kNone;
-    const uint64_t mask = __ballot(p);
-    if (mask == 0) continue;
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t at = 0;
-    if (lane == leader) at = atomicAdd(count, uint32_t(__popcll(mask)));
-    at = __shfl(at, leader) + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-    if (p) {
-      list[at] = uint32_t(v);
-      new_id[v] = at;
-    }
-  }
+__global__ __launch_bounds__(kBlock) void k_alive_list(const uint32_t* __restrict__ lvl, uint32_t n,
+                                                       uint32_t* list, uint32_t* new_id, uint32_t* count) {
+  tile_compact(
+      n, [&](uint64_t v) { return lvl[v] == kNone; },
+      [&](uint32_t at, uint64_t v) {
+        list[at] = uint32_t(v);
+        new_id[v] = at;
+      },
+      count);
 }
 
 __global__ void k_sub_degrees(const uint32_t* __restrict__ list, const uint32_t* __restrict__ deg,
@@ -409,19 +559,66 @@ __global__ void k_sub_degrees(const uint32_t* __restrict__ list, const uint32_t*
   }
 }
 
+// Heavy rows of the cut: all blocks share each row; the output order inside
+// a row is free (the vertex peel does not need sorted rows), so positions come
+// from a per-row cursor.
+__global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ adj,
+                                                           const uint32_t* __restrict__ lvl,
+                                                           const uint32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ new_id,
+                                                           const uint64_t* __restrict__ sub_off,
+                                                           uint32_t* sub_adj, const uint32_t* heavy,
+                                                           const uint32_t* heavy_count,
+                                                           const uint32_t* __restrict__ slices,
+                                                           unsigned long long* cursor) {
+  const uint32_t hc = *heavy_count;
+  if (hc == 0) return;
+  const uint32_t total = slices[hc];
+  const int lane = lane64();
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const uint32_t h = slice_row(slices, hc, t);
+    const uint32_t i = heavy[h];
+    const uint32_t v = list[i];
+    const uint64_t b = off[v] + uint64_t(t - slices[h]) * kSlice;
+    const uint64_t e = min(off[v + 1], b + kSlice);
+    for (uint64_t j0 = b + (threadIdx.x & ~63u); j0 < e; j0 += blockDim.x) {
+      const uint64_t j = j0 + lane;
+      uint32_t w = 0;
+      bool p = false;
+      if (j < e) {
+        w = adj[j];
+        p = lvl[w] == kNone;
+      }
+      const uint64_t mask = __ballot(p);
+      if (!mask) continue;
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      unsigned long long at = 0;
+      if (lane == leader) at = atomicAdd(&cursor[h], (unsigned long long)__popcll(mask));
+      at = __shfl(at, leader);
+      if (p) sub_adj[sub_off[i] + at + uint64_t(__popcll(mask & ((1ull << lane) - 1ull)))] = new_id[w];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict__ off,
                                                      const uint32_t* __restrict__ adj,
                                                      const uint32_t* __restrict__ lvl,
                                                      const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ new_id, uint32_t na,
                                                      const uint64_t* __restrict__ sub_off,
-                                                     uint32_t* sub_adj) {
+                                                     uint32_t* sub_adj, uint32_t* heavy,
+                                                     uint32_t* heavy_count) {
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   const int lane = lane64();
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < na; i += waves) {
     const uint32_t v = list[i];
     uint64_t pos = sub_off[i];
     const uint64_t e = off[v + 1];
+    if (e - off[v] > kHeavyRow) {
+      if (lane == 0) push_heavy(i, heavy, heavy_count);
+      continue;
+    }
     for (uint64_t j0 = off[v]; j0 < e; j0 += kWave) {
       const uint64_t j = j0 + lane;
       uint32_t w = 0;
@@ -642,14 +839,12 @@ __global__ __launch_bounds__(kBlock) void k_support_edges(const uint32_t* __rest
 
 // ------------------------------------------------------------ (d) peel ---
 
-__global__ void k_edge_frontier(const uint32_t* __restrict__ state, const int32_t* __restrict__ sup,
-                                uint32_t mc, int32_t thr, uint32_t* q, uint32_t* qcount) {
-  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < mc;
-       base += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t e = base + threadIdx.x;
-    const bool p = e < mc && state[e] == kNone && sup[e] < thr;
-    wave_append<uint32_t>(p, uint32_t(e), q, qcount);
-  }
+__global__ __launch_bounds__(kBlock) void k_edge_frontier(const uint32_t* __restrict__ state,
+                                                          const int32_t* __restrict__ sup, uint32_t mc,
+                                                          int32_t thr, uint32_t* q, uint32_t* qcount) {
+  tile_compact(
+      mc, [&](uint64_t e) { return state[e] == kNone && sup[e] < thr; },
+      [&](uint32_t at, uint64_t e) { q[at] = uint32_t(e); }, qcount);
 }
 
 __global__ void k_edge_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t r, uint32_t* state) {
@@ -710,14 +905,21 @@ __global__ __launch_bounds__(kBlock) void k_peel(
   }
 }
 
-__global__ void k_min_support(const uint32_t* __restrict__ state, const int32_t* __restrict__ sup,
-                              uint32_t mc, uint32_t* out) {
+__global__ __launch_bounds__(kBlock) void k_min_support(const uint32_t* __restrict__ state,
+                                                        const int32_t* __restrict__ sup, uint32_t mc,
+                                                        uint32_t* out) {
+  __shared__ uint32_t s_min[kBlock / kWave];
   uint32_t best = kNone;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < mc; e += gridDim.x * blockDim.x)
     if (state[e] == kNone) best = min(best, uint32_t(sup[e]));
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) best = min(best, uint32_t(__shfl_xor(best, o)));
-  if (lane64() == 0 && best != kNone) atomicMin(out, best);
+  if (lane64() == 0) s_min[threadIdx.x / kWave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / kWave; ++k) best = min(best, s_min[k]);
+    if (best != kNone) atomicMin(out, best);
+  }
 }
 
 __global__ void k_alive_pairs(const uint32_t* __restrict__ state, uint32_t mc, uint32_t floor,
@@ -765,6 +967,142 @@ __global__ __launch_bounds__(kBlock) void k_sons(const uint32_t* __restrict__ of
       else
         rson[eid_u[j]] = j;
     }
+}
+
+// ------------------------------------------------ dense cores (bitmaps) ---
+// For a core dense enough that a bitmap row (nc bits) is no larger than its
+// sorted row, neighbourhoods are rows of a bit matrix: bits0 (all core edges,
+// static), bits (edges still alive) and pref0 (popcount of bits0 before each
+// word, i.e. the slot of a neighbour in the sorted undirected row).
+
+__global__ __launch_bounds__(kBlock) void k_dense_rows(const uint32_t* __restrict__ off_u,
+                                                       const uint32_t* __restrict__ adj_u, uint32_t nc,
+                                                       uint32_t ws, unsigned long long* bits0,
+                                                       unsigned long long* bits, uint32_t* pref0) {
+  extern __shared__ unsigned long long row[];
+  __shared__ uint32_t wsum[kBlock / kWave];
+  for (uint32_t u = blockIdx.x; u < nc; u += gridDim.x) {
+    for (uint32_t i = threadIdx.x; i < ws; i += blockDim.x) row[i] = 0ull;
+    __syncthreads();
+    for (uint32_t j = off_u[u] + threadIdx.x; j < off_u[u + 1]; j += blockDim.x) {
+      const uint32_t w = adj_u[j];
+      atomicOr(&row[w >> 6], 1ull << (w & 63));
+    }
+    __syncthreads();
+    const uint64_t base = uint64_t(u) * ws;
+    // exclusive prefix of popcounts over the row, in chunks of blockDim words
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < ws; c0 += blockDim.x) {
+      const uint32_t i = c0 + threadIdx.x;
+      const unsigned long long wv = i < ws ? row[i] : 0ull;
+      const uint32_t pc = uint32_t(__popcll(wv));
+      uint32_t inc = pc;
+      const int lane = lane64();
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+      }
+      if (lane == kWave - 1) wsum[threadIdx.x / kWave] = inc;
+      __syncthreads();
+      uint32_t before = carry;
+      for (uint32_t k = 0; k < threadIdx.x / kWave; ++k) before += wsum[k];
+      uint32_t total = carry;
+      for (uint32_t k = 0; k < blockDim.x / kWave; ++k) total += wsum[k];
+      if (i < ws) {
+        bits0[base + i] = wv;
+        bits[base + i] = wv;
+        pref0[base + i] = before + inc - pc;
+      }
+      carry = total;
+      __syncthreads();
+    }
+  }
+}
+
+// sup(e) = |N(u) ∩ N(v)| = popcount(bits[u] & bits[v]) for every live
+// oriented edge (all edges when state is null); G lanes per edge.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_dense_support(const uint32_t* __restrict__ src,
+                                                          const uint32_t* __restrict__ dst, uint32_t mc,
+                                                          const unsigned long long* __restrict__ bits,
+                                                          uint32_t ws, const uint32_t* __restrict__ state,
+                                                          int32_t* sup) {
+  const uint32_t groups = gridDim.x * (blockDim.x / G);
+  const uint32_t lane = threadIdx.x % G;
+  for (uint32_t e = blockIdx.x * (blockDim.x / G) + threadIdx.x / G; e < mc; e += groups) {
+    if (state && state[e] != kNone) continue;  // group-uniform
+    const unsigned long long* a = bits + uint64_t(src[e]) * ws;
+    const unsigned long long* b = bits + uint64_t(dst[e]) * ws;
+    uint32_t c = 0;
+    for (uint32_t i = lane; i < ws; i += G) c += uint32_t(__popcll(a[i] & b[i]));
+    c = group_sum<G>(c);
+    if (lane == 0) sup[e] = int32_t(c);
+  }
+}
+
+__device__ __forceinline__ uint32_t dense_slot(const unsigned long long* __restrict__ bits0,
+                                               const uint32_t* __restrict__ pref0, uint64_t rowbase,
+                                               uint32_t w) {
+  const uint32_t i = w >> 6;
+  const unsigned long long below = bits0[rowbase + i] & ((1ull << (w & 63)) - 1ull);
+  return pref0[rowbase + i] + uint32_t(__popcll(below));
+}
+
+// Frontier processing on bitmaps: the common neighbours w of a frontier edge
+// (u, v) are the set bits of bits[u] & bits[v] (edges deleted in earlier
+// rounds are already cleared); the partner edge ids come from the slot of w
+// in the sorted rows. Same round-synchronous rule as k_peel.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_dense_peel(
+    const uint32_t* __restrict__ q, const uint32_t* qcount, uint32_t r, int32_t thr,
+    const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+    const unsigned long long* __restrict__ bits0, const uint32_t* __restrict__ pref0,
+    const unsigned long long* __restrict__ bits, uint32_t ws, const uint32_t* __restrict__ off_u,
+    const uint32_t* __restrict__ eid_u, const uint32_t* __restrict__ state, int32_t* sup,
+    uint32_t* qnext, uint32_t* qnext_count) {
+  const uint32_t cnt = *qcount;
+  const uint32_t groups = gridDim.x * (blockDim.x / G);
+  const uint32_t lane = threadIdx.x % G;
+  for (uint32_t qi = blockIdx.x * (blockDim.x / G) + threadIdx.x / G; qi < cnt; qi += groups) {
+    const uint32_t e = q[qi];
+    const uint32_t u = src[e], v = dst[e];
+    const uint64_t ru = uint64_t(u) * ws, rv = uint64_t(v) * ws;
+    const uint32_t ou = off_u[u], ov = off_u[v];
+    for (uint32_t i = lane; i < ws; i += G) {
+      unsigned long long t = bits[ru + i] & bits[rv + i];
+      while (t) {
+        const uint32_t w = (i << 6) + uint32_t(__ffsll(t) - 1);
+        t &= t - 1ull;
+        const uint32_t ea = eid_u[ou + dense_slot(bits0, pref0, ru, w)];
+        const uint32_t eb = eid_u[ov + dense_slot(bits0, pref0, rv, w)];
+        const uint32_t ma = state[ea], mb = state[eb];
+        if (ma < r || mb < r) continue;
+        const bool ina = (ma == r), inb = (mb == r);
+        if (!ina && !inb) {
+          peel_dec(ea, thr, sup, qnext, qnext_count);
+          peel_dec(eb, thr, sup, qnext, qnext_count);
+        } else if (ina && !inb) {
+          if (e < ea) peel_dec(eb, thr, sup, qnext, qnext_count);
+        } else if (!ina && inb) {
+          if (e < eb) peel_dec(ea, thr, sup, qnext, qnext_count);
+        }
+      }
+    }
+  }
+}
+
+// Clear the bits of the edges of a (stamped) frontier in the live bitmap.
+__global__ void k_dense_clear(const uint32_t* __restrict__ q, const uint32_t* qcount,
+                              const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                              unsigned long long* bits, uint32_t ws) {
+  const uint32_t cnt = *qcount;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const uint32_t e = q[i];
+    const uint32_t u = src[e], v = dst[e];
+    atomicAnd(&bits[uint64_t(u) * ws + (v >> 6)], ~(1ull << (v & 63)));
+    atomicAnd(&bits[uint64_t(v) * ws + (u >> 6)], ~(1ull << (u & 63)));
+  }
 }
 
 // ---------------------------------------------------- (d3) compaction ---
@@ -881,6 +1219,60 @@ void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, co
     k_w_stats<<<grid_for(mc), kBlock, 0, s>>>(off_d, src, dst, del_thr, mc, floor,
                                               reinterpret_cast<unsigned long long*>(out2));
 }
+uint32_t dense_words(uint32_t nc) { return (nc + 63) / 64; }
+bool dense_fits(uint32_t nc, uint32_t mc) {
+  if (nc == 0 || nc > 65536) return false;
+  // a bitmap row (ws words of 8 bytes) no larger than two average sorted rows
+  return uint64_t(dense_words(nc)) * nc <= 2ull * mc;
+}
+static int dense_lanes(uint32_t ws) {
+  if (ws >= 48) return 64;
+  if (ws >= 24) return 32;
+  if (ws >= 8) return 16;
+  return 8;
+}
+void dense_rows(const uint32_t* off_u, const uint32_t* adj_u, uint32_t nc, uint32_t ws, uint64_t* bits0,
+                uint64_t* bits, uint32_t* pref0, hipStream_t s) {
+  if (!nc) return;
+  k_dense_rows<<<nc < 65536 ? nc : 65536, kBlock, sizeof(uint64_t) * ws, s>>>(
+      off_u, adj_u, nc, ws, reinterpret_cast<unsigned long long*>(bits0),
+      reinterpret_cast<unsigned long long*>(bits), pref0);
+}
+void dense_support(const uint32_t* src, const uint32_t* dst, uint32_t mc, const uint64_t* bits, uint32_t ws,
+                   const uint32_t* state, int32_t* sup, hipStream_t s) {
+  if (!mc) return;
+  auto b = reinterpret_cast<const unsigned long long*>(bits);
+  switch (dense_lanes(ws)) {
+    case 8: k_dense_support<8><<<grid_for(uint64_t(mc) * 8), kBlock, 0, s>>>(src, dst, mc, b, ws, state, sup); break;
+    case 16: k_dense_support<16><<<grid_for(uint64_t(mc) * 16), kBlock, 0, s>>>(src, dst, mc, b, ws, state, sup); break;
+    case 32: k_dense_support<32><<<grid_for(uint64_t(mc) * 32), kBlock, 0, s>>>(src, dst, mc, b, ws, state, sup); break;
+    default: k_dense_support<64><<<grid_for(uint64_t(mc) * 64), kBlock, 0, s>>>(src, dst, mc, b, ws, state, sup); break;
+  }
+}
+void dense_peel(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r, int32_t thr,
+                const uint32_t* src, const uint32_t* dst, const uint64_t* bits0, const uint32_t* pref0,
+                const uint64_t* bits, uint32_t ws, const uint32_t* off_u, const uint32_t* eid_u,
+                const uint32_t* state, int32_t* sup, uint32_t* qnext, uint32_t* qnext_count, hipStream_t s) {
+  if (!maxq) return;
+  auto b0 = reinterpret_cast<const unsigned long long*>(bits0);
+  auto b = reinterpret_cast<const unsigned long long*>(bits);
+#define KT_DENSE_PEEL(G)                                                                                    \
+  k_dense_peel<G><<<grid_for(uint64_t(maxq) * G), kBlock, 0, s>>>(q, qcount, r, thr, src, dst, b0, pref0, b, ws, \
+                                                                  off_u, eid_u, state, sup, qnext, qnext_count)
+  switch (dense_lanes(ws)) {
+    case 8: KT_DENSE_PEEL(8); break;
+    case 16: KT_DENSE_PEEL(16); break;
+    case 32: KT_DENSE_PEEL(32); break;
+    default: KT_DENSE_PEEL(64); break;
+  }
+#undef KT_DENSE_PEEL
+}
+void dense_clear(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, const uint32_t* src, const uint32_t* dst,
+                 uint64_t* bits, uint32_t ws, hipStream_t s) {
+  if (maxq)
+    k_dense_clear<<<grid_for(maxq), kBlock, 0, s>>>(q, qcount, src, dst, reinterpret_cast<unsigned long long*>(bits),
+                                                   ws);
+}
 void row_src(const uint32_t* off, uint32_t rows, uint32_t* src, hipStream_t s) {
   if (rows) k_row_src<<<grid_for(uint64_t(rows) * kWave), kBlock, 0, s>>>(off, rows, src);
 }
@@ -990,7 +1382,7 @@ void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStr
 
 void core_frontier_init(const uint32_t* lvl, const uint32_t* deg, uint32_t n, uint32_t k, uint32_t* q,
                         uint32_t* qcount, hipStream_t s) {
-  if (n) k_core_frontier<<<grid_for(n), kBlock, 0, s>>>(lvl, deg, n, k, q, qcount);
+  if (n) k_core_frontier<<<grid_tiles(n), kBlock, 0, s>>>(lvl, deg, n, k, q, qcount);
 }
 void core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t k, uint32_t* lvl,
                 hipStream_t s) {
@@ -998,14 +1390,23 @@ void core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32
 }
 void core_process(const uint64_t* off, const uint32_t* adj, const uint32_t* q, const uint32_t* qcount,
                   uint32_t maxq, uint32_t k, uint32_t* lvl, uint32_t* deg, uint32_t* qnext,
-                  uint32_t* qnext_count, hipStream_t s) {
-  if (maxq)
-    k_core_process<<<grid_for(uint64_t(maxq) * kWave), kBlock, 0, s>>>(off, adj, q, qcount, k, lvl,
-                                                                       deg, qnext, qnext_count);
+                  uint32_t* qnext_count, HeavyList hl, hipStream_t s) {
+  if (!maxq) return;
+  fill_u32(hl.count, 1, 0, s);
+  k_core_process<<<grid_for(uint64_t(maxq) * kWave), kBlock, 0, s>>>(off, adj, q, qcount, k, lvl, deg, qnext,
+                                                                     qnext_count, hl.items, hl.count);
+  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices);
+  k_core_process_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, hl.items, hl.count, hl.slices, k, lvl, deg, qnext,
+                                                     qnext_count);
 }
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, uint32_t n,
-                        uint32_t* deg, hipStream_t s) {
-  if (n) k_core_alive_deg<<<grid_for(uint64_t(n) * kWave), kBlock, 0, s>>>(off, adj, lvl, n, deg);
+                        uint32_t* deg, HeavyList hl, hipStream_t s) {
+  if (!n) return;
+  fill_u32(hl.count, 1, 0, s);
+  k_core_alive_deg<<<std::min(grid_for(uint64_t(n) * kWave), 8192), kBlock, 0, s>>>(off, adj, lvl, n, deg,
+                                                                                    hl.items, hl.count);
+  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices);
+  k_core_alive_deg_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, lvl, hl.items, hl.count, hl.slices, deg);
 }
 void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s) {
   if (n) k_core_reopen_above<<<grid_for(n), kBlock, 0, s>>>(lvl, n, k);
@@ -1013,7 +1414,7 @@ void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s) {
 
 void core_collect(const uint32_t* lvl, const uint32_t* deg, const uint32_t* to_orig, uint32_t n,
                   uint64_t* keys, uint32_t* vals, uint32_t* count, hipStream_t s) {
-  if (n) k_core_collect<<<grid_for(n), kBlock, 0, s>>>(lvl, deg, to_orig, n, keys, vals, count);
+  if (n) k_core_collect<<<grid_tiles(n), kBlock, 0, s>>>(lvl, deg, to_orig, n, keys, vals, count);
 }
 void core_assign_rank(const uint64_t* keys, const uint32_t* vals, uint32_t nc, uint32_t* to, uint32_t* inv,
                       uint32_t* inv_orig, hipStream_t s) {
@@ -1021,17 +1422,23 @@ void core_assign_rank(const uint64_t* keys, const uint32_t* vals, uint32_t nc, u
 }
 void alive_list(const uint32_t* lvl, uint32_t n, uint32_t* list, uint32_t* new_id, uint32_t* count,
                 hipStream_t s) {
-  if (n) k_alive_list<<<grid_for(n), kBlock, 0, s>>>(lvl, n, list, new_id, count);
+  if (n) k_alive_list<<<grid_tiles(n), kBlock, 0, s>>>(lvl, n, list, new_id, count);
 }
 void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_orig, uint32_t na,
                  uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s) {
   if (na) k_sub_degrees<<<grid_for(na), kBlock, 0, s>>>(list, deg, to_orig, na, sub_deg, sub_orig);
 }
 void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, const uint32_t* list,
-              const uint32_t* new_id, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, hipStream_t s) {
-  if (na)
-    k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, lvl, list, new_id, na, sub_off,
-                                                                 sub_adj);
+              const uint32_t* new_id, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl,
+              uint64_t* cursor, uint32_t cursor_cap, hipStream_t s) {
+  if (!na) return;
+  fill_u32(hl.count, 1, 0, s);
+  fill_u64(cursor, cursor_cap, 0, s);
+  k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, lvl, list, new_id, na, sub_off, sub_adj,
+                                                               hl.items, hl.count);
+  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, list, hl.slices);
+  k_sub_fill_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, lvl, list, new_id, sub_off, sub_adj, hl.items, hl.count,
+                                                 hl.slices, reinterpret_cast<unsigned long long*>(cursor));
 }
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to, const uint32_t* inv,
                   uint32_t nc, uint32_t* dout, hipStream_t s) {
@@ -1093,7 +1500,7 @@ void support_edges(const uint32_t* off_d, const uint32_t* adj_d, const uint32_t*
 
 void edge_frontier_init(const uint32_t* state, const int32_t* sup, uint32_t mc, int32_t thr, uint32_t* q,
                         uint32_t* qcount, hipStream_t s) {
-  if (mc) k_edge_frontier<<<grid_for(mc), kBlock, 0, s>>>(state, sup, mc, thr, q, qcount);
+  if (mc) k_edge_frontier<<<grid_tiles(mc), kBlock, 0, s>>>(state, sup, mc, thr, q, qcount);
 }
 void edge_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r, uint32_t* state,
                 hipStream_t s) {
@@ -1124,7 +1531,7 @@ void peel_process(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* 
   }
 }
 void edge_min_support(const uint32_t* state, const int32_t* sup, uint32_t mc, uint32_t* out, hipStream_t s) {
-  if (mc) k_min_support<<<grid_for(mc), kBlock, 0, s>>>(state, sup, mc, out);
+  if (mc) k_min_support<<<std::min(grid_for(mc), 2048), kBlock, 0, s>>>(state, sup, mc, out);
 }
 void edges_alive_above(const uint32_t* state, uint32_t mc, uint32_t stamp_floor, const uint32_t* src,
                        const uint32_t* dst, const uint32_t* inv, uint32_t* pairs_out, uint32_t* count,

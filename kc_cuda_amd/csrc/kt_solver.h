@@ -48,6 +48,11 @@ struct CoreView {
   uint32_t mc0 = 0, max_dout0 = 0;
   uint32_t *off_d0 = nullptr, *adj_d0 = nullptr, *src0 = nullptr, *del_thr = nullptr;
   int work = -1;  // which work set holds the current edges (-1: base)
+  // dense cores: bit-matrix rows (ws words each) of all / live edges
+  bool dense = false;
+  uint32_t ws = 0;
+  uint64_t *bits0 = nullptr, *bits = nullptr;
+  uint32_t* pref0 = nullptr;
 };
 
 struct TrussOutcome {
@@ -82,6 +87,7 @@ class Solver {
   void set_lanes(int count_lanes, int peel_lanes) { lanes_count_ = count_lanes; lanes_peel_ = peel_lanes; }
   void set_force_edge_count(bool v) { force_edge_count_ = v; }
   void set_force_atomic_build(bool v) { force_atomic_build_ = v; }
+  void set_core_mode(int m) { core_mode_ = m; }  // 0 auto, 1 sparse, 2 dense
 
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }
@@ -146,6 +152,16 @@ class Solver {
   int lanes_count_ = 0, lanes_peel_ = 0;
   bool force_edge_count_ = false;
   bool force_atomic_build_ = false;
+  int core_mode_ = 0;
+  DevBuf bits0_, bits_, pref0_, heavy_;
+  double dense_tri_cost_ = 8.0, sparse_drop_ratio_ = 3.0;
+  bool trace_ = false;  // KT_TRACE=1: per-round frontier sizes on stderr
+  void make_dense(CoreView* cv);
+  bool drop_frontier(const CoreView& cv, uint64_t c, uint64_t alive, int32_t thr) const;
+  HeavyList heavy() {
+    heavy_.ensure(sizeof(uint32_t) * 2 * (size_t(n_) + 2));
+    return HeavyList{heavy_.as<uint32_t>(), &ctl_->pad[1], heavy_.as<uint32_t>() + size_t(n_) + 2};
+  }
 
   uint32_t n_ = 0;
   uint64_t m_ = 0;

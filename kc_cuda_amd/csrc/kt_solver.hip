@@ -42,6 +42,10 @@ Solver::Solver(int device) : device_(device) {
   KT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 64, hipHostMallocDefault));
   for (auto& e : ev_) KT_HIP(hipEventCreate(&e));
   KT_HIP(hipStreamSynchronize(stream_));
+  // cost-model knobs of the frontier-versus-recount choice (tuning only)
+  if (const char* v = std::getenv("KT_DENSE_TRI_COST")) dense_tri_cost_ = std::atof(v);
+  if (const char* v = std::getenv("KT_SPARSE_DROP_RATIO")) sparse_drop_ratio_ = std::atof(v);
+  if (const char* v = std::getenv("KT_TRACE")) trace_ = v[0] == '1';
 }
 
 Solver::~Solver() {
@@ -51,7 +55,7 @@ Solver::~Solver() {
                    &keys_, &keys2_, &vals_, &vals2_, &to_, &inv_, &inv_orig_, &dout_, &off_d_,
                    &adj_d_, &src_, &din_, &din_off_, &tkeys_, &tvals_, &tiota_, &in_src_, &off_u_,
                    &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &del_thr_, &orig0_,
-                   &temp_, &hist_, &hoff_, &part_};
+                   &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &pref0_, &heavy_};
   for (auto* b : all) b->release();
   for (auto& w : work_) {
     w.off.release();

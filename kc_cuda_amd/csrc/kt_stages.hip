@@ -48,9 +48,10 @@ uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
   uint64_t removed = 0;
   int cur = 0;
   while (c) {
+    if (trace_) std::fprintf(stderr, "[kt] vertex-peel level_n=%u k=%u frontier=%u\n", g.n, k, c);
     core_stamp(q[cur], &qc[cur], c, k, g.lvl, stream_);
     zero_u32(&qc[cur ^ 1]);
-    core_process(g.off, g.adj, q[cur], &qc[cur], c, k, g.lvl, g.deg, q[cur ^ 1], &qc[cur ^ 1], stream_);
+    core_process(g.off, g.adj, q[cur], &qc[cur], c, k, g.lvl, g.deg, q[cur ^ 1], &qc[cur ^ 1], heavy(), stream_);
     removed += c;
     cur ^= 1;
     c = read_u32(&qc[cur]);
@@ -85,7 +86,10 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
   s.slots = read_u64(s.off + na);
   s.b_adj.ensure(sizeof(uint32_t) * std::max<uint64_t>(s.slots, 1));
   s.adj = s.b_adj.as<uint32_t>();
-  sub_fill(p.off, p.adj, p.lvl, list_.as<uint32_t>(), new_id_.as<uint32_t>(), na, s.off, s.adj, stream_);
+  const uint32_t cap = uint32_t(std::min<uint64_t>(na, p.slots / 8192 + 1));
+  cursor_.ensure(sizeof(uint64_t) * (size_t(cap) + 1));
+  sub_fill(p.off, p.adj, p.lvl, list_.as<uint32_t>(), new_id_.as<uint32_t>(), na, s.off, s.adj, heavy(),
+           cursor_.as<uint64_t>(), cap, stream_);
   fill_u32(s.lvl, na, kNone, stream_);
   ++depth_;
   return s;
@@ -149,7 +153,7 @@ void Solver::core_at(uint32_t k) {
   for (uint32_t t : g.ok)
     if (t <= k) kj = std::max(kj, t);
   core_reopen_above(g.lvl, g.n, kj, stream_);
-  core_alive_degrees(g.off, g.adj, g.lvl, g.n, g.deg, stream_);
+  core_alive_degrees(g.off, g.adj, g.lvl, g.n, g.deg, heavy(), stream_);
   if (k > kj) peel_vertices(g, k);
 }
 
@@ -224,6 +228,23 @@ void Solver::extract_core(LevelGraph& g, CoreView* cv) {
   cv->state = state_.as<uint32_t>();
   cv->work = -1;
   build_undirected_view(cv);
+  if (core_mode_ == 2 ? cv->nc <= 65536 : (core_mode_ == 0 && dense_fits(cv->nc, cv->mc))) make_dense(cv);
+}
+
+// Bit-matrix rows of the core (all edges, live edges) plus per-word prefix
+// popcounts of the static rows.
+void Solver::make_dense(CoreView* cv) {
+  const uint32_t ws = dense_words(cv->nc);
+  const size_t words = size_t(cv->nc) * ws;
+  bits0_.ensure(sizeof(uint64_t) * std::max<size_t>(words, 1));
+  bits_.ensure(sizeof(uint64_t) * std::max<size_t>(words, 1));
+  pref0_.ensure(sizeof(uint32_t) * std::max<size_t>(words, 1));
+  cv->dense = true;
+  cv->ws = ws;
+  cv->bits0 = bits0_.as<uint64_t>();
+  cv->bits = bits_.as<uint64_t>();
+  cv->pref0 = pref0_.as<uint32_t>();
+  dense_rows(cv->off_u, cv->adj_u, cv->nc, ws, cv->bits0, cv->bits, cv->pref0, stream_);
 }
 
 // Undirected CSR of the current oriented edges: row w = [sources of w's
@@ -270,7 +291,9 @@ void Solver::count_support(CoreView* cv, float* ms) {
   const double avg = cv->nc ? double(cv->mc) / cv->nc : 0.0;
   const int lanes = lanes_count_ ? lanes_count_ : auto_lanes(avg);
   KT_HIP(hipEventRecord(ev_[8], stream_));
-  if (!force_edge_count_ && support_bitmap_fits(cv->nc, cv->max_dout))
+  if (cv->dense)
+    dense_support(cv->src, cv->adj_d, cv->mc, cv->bits, cv->ws, nullptr, cv->sup, stream_);
+  else if (!force_edge_count_ && support_bitmap_fits(cv->nc, cv->max_dout))
     support_bitmap(cv->off_d, cv->adj_d, cv->nc, cv->max_dout, lanes, cv->sup, ctl_, stream_);
   else
     support_edges(cv->off_d, cv->adj_d, cv->src, cv->nc, cv->mc, lanes, cv->sup, stream_);
@@ -317,6 +340,17 @@ void Solver::compact_core(CoreView* cv, bool carry_sup) {
 
 // ------------------------------------------------------------- (d) peel ---
 
+// Whether a frontier of c edges (supports < thr) is cheaper to drop with an
+// exact recount of the alive edges than to walk triangle by triangle. Dense
+// cores: a recount is alive * ws word ANDs, a frontier walk c * (ws + ~8 thr)
+// random lookups and atomics, so a few percent of the alive edges already
+// favour the recount. Sorted-row cores pay a compaction and a rebuild first:
+// the reference's third (src/kc.cu:988) scaled by recount_ratio_.
+bool Solver::drop_frontier(const CoreView& cv, uint64_t c, uint64_t alive, int32_t thr) const {
+  if (cv.dense) return double(c) * (cv.ws + dense_tri_cost_ * double(thr)) >= double(alive) * cv.ws;
+  return double(c) * sparse_drop_ratio_ >= double(alive);
+}
+
 // Truss levels on the core view, upward from threshold thr0 (an edge stays
 // while its support >= thr, i.e. in the (thr+2)-truss). Empty levels are
 // skipped by jumping to the minimum live support. With early_ans != 0, a
@@ -340,13 +374,16 @@ TrussOutcome Solver::truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans
     return read_u32(&qc[0]);
   };
   for (;;) {
-    if (2 * alive <= cv->mc) {  // half the edges are dead: compact, keep supports
+    if (!cv->dense && 2 * alive <= cv->mc) {  // half the edges are dead: compact, keep supports
       compact_core(cv, true);
       ++out.compactions;
       r = 1;
     }
     uint32_t c = frontier();
     while (c) {
+      if (trace_)
+        std::fprintf(stderr, "[kt] edge-peel thr=%d alive=%llu mc=%u frontier=%u dense=%d drop=%d\n", thr,
+                     (unsigned long long)alive, cv->mc, c, int(cv->dense), int(drop_frontier(*cv, c, alive, thr)));
       edge_stamp_thr(q[cur], &qc[cur], c, r, uint32_t(thr), cv->state, cv->orig, cv->del_thr, stream_);
       ++out.rounds;
       if (c == alive) {  // every live edge goes: the level is empty
@@ -354,24 +391,37 @@ TrussOutcome Solver::truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans
         alive = 0;
         break;
       }
-      if (3ull * c >= alive) {  // drop the frontier and recount the rest exactly
-        compact_core(cv, false);
+      if (drop_frontier(*cv, c, alive, thr)) {  // drop the frontier, recount the rest exactly
+        if (cv->dense) {
+          dense_clear(q[cur], &qc[cur], c, cv->src, cv->adj_d, cv->bits, cv->ws, stream_);
+          dense_support(cv->src, cv->adj_d, cv->mc, cv->bits, cv->ws, cv->state, cv->sup, stream_);
+          alive -= c;
+          ++r;
+        } else {
+          compact_core(cv, false);
+          alive = cv->mc;
+          r = 1;
+        }
         ++out.compactions;
-        alive = cv->mc;
-        r = 1;
         c = frontier();
         continue;
       }
-      const double avg = cv->nc ? double(2ull * cv->mc) / cv->nc : 0.0;
-      const int lanes = lanes_peel_ ? lanes_peel_ : auto_lanes(avg);
       zero_u32(&qc[cur ^ 1]);
-      peel_process(cv->off_u, cv->adj_u, cv->eid_u, cv->src, cv->adj_d, q[cur], &qc[cur], c, r, thr, lanes,
-                   cv->state, cv->sup, q[cur ^ 1], &qc[cur ^ 1], stream_);
+      if (cv->dense) {
+        dense_peel(q[cur], &qc[cur], c, r, thr, cv->src, cv->adj_d, cv->bits0, cv->pref0, cv->bits, cv->ws,
+                   cv->off_u, cv->eid_u, cv->state, cv->sup, q[cur ^ 1], &qc[cur ^ 1], stream_);
+        dense_clear(q[cur], &qc[cur], c, cv->src, cv->adj_d, cv->bits, cv->ws, stream_);
+      } else {
+        const double avg = cv->nc ? double(2ull * cv->mc) / cv->nc : 0.0;
+        const int lanes = lanes_peel_ ? lanes_peel_ : auto_lanes(avg);
+        peel_process(cv->off_u, cv->adj_u, cv->eid_u, cv->src, cv->adj_d, q[cur], &qc[cur], c, r, thr, lanes,
+                     cv->state, cv->sup, q[cur ^ 1], &qc[cur ^ 1], stream_);
+      }
       alive -= c;
       ++r;
       cur ^= 1;
       c = read_u32(&qc[cur]);
-      if (c && 2 * alive <= cv->mc) {
+      if (!cv->dense && c && 2 * alive <= cv->mc) {
         compact_core(cv, true);
         ++out.compactions;
         if (cv->mc != alive) fail("edge peel bookkeeping mismatch");

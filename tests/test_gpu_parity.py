@@ -72,6 +72,30 @@ def test_kernel_variants_agree(ctx, lanes, force_edge):
         ctx.tune(0, 0, False)
 
 
+@pytest.mark.parametrize("core", ["sparse", "dense"])
+@pytest.mark.parametrize("seed", range(8))
+def test_core_representations_agree(ctx, core, seed):
+    """Sorted-row and bit-matrix cores give the same kmax-truss."""
+    from kc_cuda_amd import gen
+    O = _oracle()
+    if seed < 4:
+        pairs = random_graph([60, 200, 500, 900][seed], [0.3, 0.1, 0.04, 0.02][seed], 40 + seed,
+                             cliques=(seed + 5, 4))
+        n = [60, 200, 500, 900][seed]
+    else:
+        pairs, n = gen.rmat(10 + seed, 16, 0.57, 0.19, 0.19, True, 6 * seed, 90 + seed)
+    print(f"[gpu] core={core} seed={seed} m={len(pairs)}", flush=True)
+    ref = O.solve_pairs(pairs, n, want_edges=True)
+    try:
+        ctx.tune(core=core)
+        r = ctx.solve_pairs(pairs, n)
+        assert (r.kmax, r.truss_edges) == (ref.kmax, ref.truss_edges)
+        assert r.k_hat == ref.k_hat
+        np.testing.assert_array_equal(ctx.truss_edges(), canon_pairs(ref.truss))
+    finally:
+        ctx.tune()
+
+
 @pytest.mark.parametrize("scale", [12, 16, 18])
 def test_rmat_vs_golden(ctx, scale):
     from kc_cuda_amd import gen
