@@ -31,7 +31,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall",
              "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
-DEVICE_SOURCES = ["kt_kernels.hip", "kt_solver.hip", "kt_stages.hip", "kt_batch.hip", "kt_compat.hip",
+DEVICE_SOURCES = ["kt_kernels.hip", "kt_peel.hip", "kt_solver.hip", "kt_stages.hip", "kt_compat.hip",
                   "kt_capi.cpp", "kt_tsv.cpp"]
 
 
@@ -58,7 +58,7 @@ def build_device(quiet: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
     LIB.mkdir(exist_ok=True)
     BIN.mkdir(exist_ok=True)
-    srcs = [CSRC / s for s in DEVICE_SOURCES if (CSRC / s).exists()]
+    srcs = [CSRC / s for s in DEVICE_SOURCES]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, quiet), srcs))
     so = LIB / "libkc_amd.so"
@@ -69,11 +69,6 @@ def build_device(quiet: bool = False) -> Path:
     cli = BIN / "kc_gpu"
     _run([HIPCC, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", "-o", str(cli),
           str(CSRC / "kc_gpu_main.cpp"), f"-L{LIB}", "-lkc_amd", "-Wl,-rpath,$ORIGIN/../lib"], quiet)
-    compat_main = CSRC / "kc_gpu_compat_main.cpp"
-    if compat_main.exists():
-        _run([HIPCC, "-O2", "-std=c++17", "-fopenmp", f"-I{ROOT / 'include'}", "-o",
-              str(BIN / "kc_gpu_compat"), str(compat_main), f"-L{LIB}", "-lkc_amd",
-              "-Wl,-rpath,$ORIGIN/../lib"], quiet)
     return so
 
 

@@ -43,19 +43,34 @@ struct Ctl {
   uint32_t nalive;       // scratch counter (compaction)
   uint32_t minsup;       // min support among live edges
   uint32_t pad[2];
-  // device-driven peel rounds: which queue is current, what this round does,
-  // the round stamp, and running totals
+  // Device-driven peel rounds. The kernels of a round read which queue holds
+  // the frontier (cur), what to do with it (action) and the round stamp from
+  // here, and the last workgroup of a round's closing kernel (ticket) decides
+  // the next round, so the host can enqueue many rounds between readbacks.
   uint32_t cur;
   uint32_t action;       // kRound* below
-  uint32_t round;
-  uint32_t rounds_done;
-  unsigned long long alive;    // live edges (edge peel)
-  unsigned long long removed;  // vertices removed (vertex peel)
+  uint32_t round;        // stamp of the current frontier
+  uint32_t rounds_done;  // frontiers stamped
+  uint32_t ticket;       // last-workgroup detection
+  uint32_t stamped;      // the current frontier has been stamped
+  unsigned long long alive;    // live, unstamped edges / alive vertices
+  unsigned long long removed;  // vertices removed by the vertex peel
 };
 
-constexpr uint32_t kRoundIdle = 0;   // empty frontier: nothing to do
-constexpr uint32_t kRoundPeel = 1;   // walk the frontier's triangles
-constexpr uint32_t kRoundDrop = 2;   // drop the frontier, recount the rest
-constexpr uint32_t kRoundEmpty = 3;  // the frontier is every live edge
+constexpr uint32_t kRoundIdle = 0;     // empty frontier: the level is done
+constexpr uint32_t kRoundPeel = 1;     // walk the frontier's triangles
+constexpr uint32_t kRoundDrop = 2;     // drop the frontier, recount the rest
+constexpr uint32_t kRoundEmpty = 3;    // the frontier is every live edge
+constexpr uint32_t kRoundCompact = 4;  // compact the edge set first (unstamped)
+
+// Host-set parameters of the device round decision (drop or walk a frontier).
+struct RoundRule {
+  uint32_t dense;     // bit-matrix core
+  uint32_t ws;        // words per bit-matrix row
+  uint32_t mc;        // edges of the current edge set (compaction rule)
+  int32_t thr;        // level threshold
+  double tri_cost;    // dense: cost of one triangle visit in word ANDs
+  double drop_ratio;  // sparse: drop when frontier * ratio >= alive
+};
 
 }  // namespace kt

@@ -174,7 +174,7 @@ void Solver::view_from_oriented_host(uint32_t nc, const uint32_t* h_off_d, const
   fill_u32(del_thr_.as<uint32_t>(), mc, kNone, stream_);
   fill_u32(state_.as<uint32_t>(), mc, kNone, stream_);
   *cv = CoreView{};
-  cv->nc = nc;
+  cv->nc = cv->nc0 = nc;
   cv->mc = cv->mc0 = mc;
   cv->max_dout = cv->max_dout0 = mx;
   cv->inv = inv_orig_.as<uint32_t>();

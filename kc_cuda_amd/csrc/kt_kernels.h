@@ -38,12 +38,14 @@ void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, uint32_t B, uin
 //     threshold at which v was removed. deg holds alive-neighbour counts.
 void core_frontier_init(const uint32_t* lvl, const uint32_t* deg, uint32_t n, uint32_t k,
                         uint32_t* q, uint32_t* qcount, hipStream_t s);
-void core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t k,
-                uint32_t* lvl, hipStream_t s);
-void core_process(const uint64_t* off, const uint32_t* adj, const uint32_t* q,
-                  const uint32_t* qcount, uint32_t maxq, uint32_t k, uint32_t* lvl,
-                  uint32_t* deg, uint32_t* qnext, uint32_t* qnext_count, HeavyList hl,
-                  hipStream_t s);
+// Device-driven rounds: core_round_start reads the frontier size from
+// ctl->qcount[0] (filled by core_frontier_init into q0); each core_round
+// stamps the frontier, decrements the alive neighbours' degrees, queues the
+// ones that fall below k and switches queues. ctl->removed counts the
+// removed vertices; ctl->action turns idle once a frontier is empty.
+void core_round_start(Ctl* ctl, hipStream_t s);
+void core_round(const uint64_t* off, const uint32_t* adj, uint32_t* q0, uint32_t* q1, Ctl* ctl,
+                uint32_t k, uint32_t* lvl, uint32_t* deg, HeavyList hl, hipStream_t s);
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl,
                         uint32_t n, uint32_t* deg, HeavyList hl, hipStream_t s);
 // lvl[v] = (lvl[v] > k) ? kNone : (lvl[v] == kNone ? kNone : k) -- used to
@@ -61,11 +63,12 @@ void alive_list(const uint32_t* lvl, uint32_t n, uint32_t* list, uint32_t* new_i
                 uint32_t* count, hipStream_t s);
 void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_orig, uint32_t na,
                  uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s);
-// cursor: cursor_cap zeroed 64-bit slots, at least the number of rows of the
-// cut longer than the heavy threshold (slots / 8192 + 1 suffices)
-void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, const uint32_t* list,
-              const uint32_t* new_id, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj,
-              HeavyList hl, uint64_t* cursor, uint32_t cursor_cap, hipStream_t s);
+// new_id from alive_list (kNone marks removed vertices). cursor: cursor_cap
+// zeroed 64-bit slots, at least the number of rows of the cut longer than the
+// heavy threshold (slots / 8192 + 1 suffices)
+void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* new_id,
+              uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl, uint64_t* cursor,
+              uint32_t cursor_cap, hipStream_t s);
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
                   const uint32_t* inv, uint32_t nc, uint32_t* dout, hipStream_t s);
 void orient_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
@@ -95,18 +98,8 @@ void support_edges(const uint32_t* off_d, const uint32_t* adj_d, const uint32_t*
 //     < r: deleted earlier.
 void edge_frontier_init(const uint32_t* state, const int32_t* sup, uint32_t mc, int32_t thr,
                         uint32_t* q, uint32_t* qcount, hipStream_t s);
-void edge_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r,
-                uint32_t* state, hipStream_t s);
-void peel_process(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u,
-                  const uint32_t* src, const uint32_t* dst, const uint32_t* q,
-                  const uint32_t* qcount, uint32_t maxq, uint32_t r, int32_t thr, int lanes,
-                  uint32_t* state, int32_t* sup, uint32_t* qnext, uint32_t* qnext_count,
-                  hipStream_t s);
 void edge_min_support(const uint32_t* state, const int32_t* sup, uint32_t mc, uint32_t* out,
                       hipStream_t s);
-void edges_alive_above(const uint32_t* state, uint32_t mc, uint32_t stamp_floor,
-                       const uint32_t* src, const uint32_t* dst, const uint32_t* inv,
-                       uint32_t* pairs_out, uint32_t* count, hipStream_t s);
 
 void w_stats(const uint32_t* off_d, const uint32_t* src, const uint32_t* dst, const uint32_t* del_thr,
              uint32_t mc, int32_t floor, uint64_t* out2, hipStream_t s);
@@ -118,13 +111,33 @@ void dense_rows(const uint32_t* off_u, const uint32_t* adj_u, uint32_t nc, uint3
                 uint64_t* bits0, uint64_t* bits, uint32_t* pref0, hipStream_t s);
 void dense_support(const uint32_t* src, const uint32_t* dst, uint32_t mc, const uint64_t* bits,
                    uint32_t ws, const uint32_t* state, int32_t* sup, hipStream_t s);
-void dense_peel(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r, int32_t thr,
-                const uint32_t* src, const uint32_t* dst, const uint64_t* bits0,
-                const uint32_t* pref0, const uint64_t* bits, uint32_t ws, const uint32_t* off_u,
-                const uint32_t* eid_u, const uint32_t* state, int32_t* sup, uint32_t* qnext,
-                uint32_t* qnext_count, hipStream_t s);
-void dense_clear(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, const uint32_t* src,
-                 const uint32_t* dst, uint64_t* bits, uint32_t ws, hipStream_t s);
+
+// Device-driven edge-peel rounds (kt_peel.hip). q0/q1 are the two frontier
+// queues; ctl->cur says which one holds the current frontier.
+void round_start(Ctl* ctl, uint32_t r, uint64_t alive, const RoundRule& rule, bool reset, hipStream_t s);
+void round_stamp(uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t thr, uint32_t* state, const uint32_t* orig,
+                 uint32_t* del_thr, const uint32_t* src, const uint32_t* dst, uint64_t* fbits, uint32_t ws,
+                 hipStream_t s);
+void round_advance(uint32_t* q0, uint32_t* q1, Ctl* ctl, const uint32_t* src, const uint32_t* dst,
+                   uint64_t* bits, uint64_t* fbits, uint32_t ws, const RoundRule& rule, hipStream_t s);
+void frontier_clear(const uint32_t* q0, const uint32_t* q1, const Ctl* ctl, const uint32_t* src,
+                    const uint32_t* dst, uint64_t* bits, uint64_t* fbits, uint32_t ws, hipStream_t s);
+void dense_peel_round(uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, const uint32_t* src,
+                      const uint32_t* dst, const uint64_t* bits, const uint64_t* fbits, uint32_t ws,
+                      const uint32_t* eidm, uint32_t nc, const uint64_t* bits0, const uint32_t* pref0,
+                      const uint32_t* off_u, const uint32_t* eid_u, int32_t* sup, hipStream_t s);
+void sparse_peel_round(uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, int lanes, const uint32_t* off_u,
+                       const uint32_t* adj_u, const uint32_t* eid_u, const uint32_t* src,
+                       const uint32_t* dst, const uint32_t* state, int32_t* sup, hipStream_t s);
+void dense_eidm(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, uint32_t nc,
+                uint32_t* eidm, hipStream_t s);
+// Vertex compaction of a core view: flag the endpoints of live edges (flag
+// zeroed first), then, with vmap = exclusive scan of the flags, renumber the
+// compacted edges and rebuild the row offsets (off gets vmap[n] + 1 entries).
+void live_vertex_flags(const uint32_t* state, const uint32_t* src, const uint32_t* dst, uint32_t mc,
+                       uint32_t* flag, hipStream_t s);
+void relabel_edges(uint32_t* adj, uint32_t* src, uint32_t m, const uint32_t* vmap, hipStream_t s);
+void relabel_offsets(const uint32_t* roff, const uint32_t* vmap, uint32_t n, uint32_t* off, hipStream_t s);
 
 void row_src(const uint32_t* off, uint32_t rows, uint32_t* src, hipStream_t s);
 void sons(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, uint32_t nc,
@@ -137,9 +150,6 @@ void row_compact(const uint32_t* off_d, const uint32_t* adj_d, const uint32_t* o
                  const int32_t* sup, const uint32_t* state, uint32_t nc, const uint32_t* off2,
                  uint32_t* adj2, uint32_t* src2, uint32_t* orig2, int32_t* sup2, hipStream_t s);
 void count_in_degrees(const uint32_t* adj, uint32_t m, uint32_t* din, hipStream_t s);
-void edge_stamp_thr(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r,
-                    uint32_t thr, uint32_t* state, const uint32_t* orig, uint32_t* del_thr,
-                    hipStream_t s);
 void truss_pairs(const uint32_t* del_thr, uint32_t m0, int32_t floor, const uint32_t* src0,
                  const uint32_t* dst0, const uint32_t* inv, uint32_t* out, uint32_t* count,
                  hipStream_t s);

@@ -10,115 +10,11 @@
 // The algorithms are restated for wave64 and LDS, not translated: frontier
 // queues replace full-array mask scans, hub rows live in LDS as a bitmap plus
 // a sorted list, and same-row credits are accumulated in LDS.
+#include "kt_device.h"
 #include "kt_kernels.h"
 
 namespace kt {
 namespace {
-
-constexpr int kMaxGrid = 1 << 20;
-
-inline int grid_for(uint64_t items, int per_block = kBlock) {
-  uint64_t g = (items + per_block - 1) / per_block;
-  if (g < 1) g = 1;
-  if (g > uint64_t(kMaxGrid)) g = kMaxGrid;
-  return int(g);
-}
-
-__device__ __forceinline__ int lane64() { return int(__lane_id()); }
-
-// Wave-aggregated queue append. Must be reached by every lane of the wave.
-template <typename T>
-__device__ __forceinline__ void wave_append(bool pred, T item, T* q, uint32_t* count) {
-  const uint64_t mask = __ballot(pred);
-  if (mask == 0) return;
-  const int lane = lane64();
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(count, uint32_t(__popcll(mask)));
-  base = __shfl(base, leader);
-  if (pred) q[base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = item;
-}
-
-// Block-wide exclusive scan of one value per thread (blockDim == kBlock).
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
-  __shared__ uint32_t s_w[kBlock / kWave];
-  const int lane = lane64(), wid = int(threadIdx.x) / kWave;
-  uint32_t inc = v;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint32_t y = __shfl_up(inc, o);
-    if (lane >= o) inc += y;
-  }
-  if (lane == kWave - 1) s_w[wid] = inc;
-  __syncthreads();
-  uint32_t before = 0, tot = 0;
-#pragma unroll
-  for (int k = 0; k < kBlock / kWave; ++k) {
-    const uint32_t w = s_w[k];
-    if (k < wid) before += w;
-    tot += w;
-  }
-  __syncthreads();
-  *total = tot;
-  return before + inc - v;
-}
-
-// Stream compaction of [0, n) by tiles of kBlock * kTileItems indices: one
-// counter atomic per tile instead of one per wave (a single queue counter hit
-// by every wave of a 4M-vertex sweep serialises the whole launch).
-constexpr int kTileItems = 16;
-template <typename Pred, typename Emit>
-__device__ __forceinline__ void tile_compact(uint64_t n, Pred pred, Emit emit, uint32_t* count) {
-  __shared__ uint32_t s_base;
-  const uint64_t tile = uint64_t(blockDim.x) * kTileItems;
-  for (uint64_t t0 = uint64_t(blockIdx.x) * tile; t0 < n; t0 += uint64_t(gridDim.x) * tile) {
-    uint32_t bits = 0;
-#pragma unroll
-    for (int k = 0; k < kTileItems; ++k) {
-      const uint64_t i = t0 + uint64_t(k) * blockDim.x + threadIdx.x;
-      if (i < n && pred(i)) bits |= 1u << k;
-    }
-    uint32_t total;
-    const uint32_t pre = block_exclusive_scan(uint32_t(__popc(bits)), &total);
-    if (threadIdx.x == 0 && total) s_base = atomicAdd(count, total);
-    __syncthreads();
-    uint32_t at = s_base + pre;
-#pragma unroll
-    for (int k = 0; k < kTileItems; ++k)
-      if ((bits >> k) & 1u) emit(at++, t0 + uint64_t(k) * blockDim.x + threadIdx.x);
-    __syncthreads();
-  }
-}
-
-inline int grid_tiles(uint64_t n) {
-  const uint64_t tile = uint64_t(kBlock) * kTileItems;
-  uint64_t g = (n + tile - 1) / tile;
-  if (g < 1) g = 1;
-  if (g > 4096) g = 4096;
-  return int(g);
-}
-
-template <int G>
-__device__ __forceinline__ uint32_t group_sum(uint32_t v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
-  return v;
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return group_sum<64>(v); }
-
-// first index in [lo, hi) with a[idx] >= x
-template <typename P>
-__device__ __forceinline__ uint32_t lower_bound_u32(const P a, uint32_t lo, uint32_t hi, uint32_t x) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] < x)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
 
 // ------------------------------------------------------------ utilities ---
 
@@ -343,10 +239,35 @@ __global__ __launch_bounds__(kBlock) void k_core_frontier(const uint32_t* __rest
       [&](uint32_t at, uint64_t v) { q[at] = uint32_t(v); }, qcount);
 }
 
-__global__ void k_core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t k, uint32_t* lvl) {
-  const uint32_t cnt = *qcount;
+// Device-driven vertex-peel rounds: the frontier is q[ctl->cur] with
+// ctl->qcount[cur] entries; every kernel of a round does nothing once a
+// round has found an empty frontier (action idle).
+__global__ void k_vround_start(Ctl* ctl) {
+  if (threadIdx.x != 0) return;
+  ctl->cur = 0;
+  ctl->qcount[1] = 0;
+  ctl->pad[1] = 0;  // heavy-row count
+  ctl->removed = 0;
+  ctl->action = ctl->qcount[0] ? kRoundPeel : kRoundIdle;
+}
+
+__global__ void k_core_stamp(const uint32_t* q0, const uint32_t* q1, const Ctl* ctl, uint32_t k, uint32_t* lvl) {
+  if (ctl->action != kRoundPeel) return;
+  const uint32_t cur = ctl->cur, cnt = ctl->qcount[cur];
+  const uint32_t* q = cur ? q1 : q0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
     lvl[q[i]] = k;
+}
+
+// Close a round: count what it removed, make the next queue current.
+__global__ void k_vround_advance(Ctl* ctl) {
+  if (threadIdx.x != 0 || ctl->action != kRoundPeel) return;
+  const uint32_t cur = ctl->cur;
+  ctl->removed += ctl->qcount[cur];
+  ctl->qcount[cur] = 0;
+  ctl->cur = cur ^ 1u;
+  ctl->pad[1] = 0;
+  ctl->action = ctl->qcount[cur ^ 1u] ? kRoundPeel : kRoundIdle;
 }
 
 // Rows longer than kHeavyRow slots are not walked by one wave: the light
@@ -367,7 +288,9 @@ constexpr uint64_t kSlice = 4096;
 __global__ __launch_bounds__(kBlock) void k_heavy_plan(const uint64_t* __restrict__ off,
                                                        const uint32_t* __restrict__ heavy,
                                                        const uint32_t* heavy_count,
-                                                       const uint32_t* __restrict__ list, uint32_t* slices) {
+                                                       const uint32_t* __restrict__ list, uint32_t* slices,
+                                                       const Ctl* gate) {
+  if (gate && gate->action != kRoundPeel) return;
   const uint32_t hc = *heavy_count;
   uint32_t carry = 0;
   for (uint32_t c0 = 0; c0 < hc; c0 += blockDim.x) {
@@ -408,9 +331,13 @@ __device__ __forceinline__ void core_visit(uint32_t w, bool valid, uint32_t k, c
 __global__ __launch_bounds__(kBlock) void k_core_process_heavy(
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj, const uint32_t* heavy,
     const uint32_t* heavy_count, const uint32_t* __restrict__ slices, uint32_t k,
-    const uint32_t* __restrict__ lvl, uint32_t* deg, uint32_t* qnext, uint32_t* qnext_count) {
+    const uint32_t* __restrict__ lvl, uint32_t* deg, uint32_t* q0, uint32_t* q1, Ctl* ctl) {
+  if (ctl->action != kRoundPeel) return;
   const uint32_t hc = *heavy_count;
   if (hc == 0) return;
+  const uint32_t cur = ctl->cur;
+  uint32_t* qnext = cur ? q0 : q1;
+  uint32_t* qnext_count = &ctl->qcount[cur ^ 1u];
   const uint32_t total = slices[hc];
   const int lane = lane64();
   for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
@@ -428,10 +355,13 @@ __global__ __launch_bounds__(kBlock) void k_core_process_heavy(
 // One wave per removed vertex: every still-alive neighbour loses one degree;
 // the decrement that takes it from k to k-1 queues it (exactly once).
 __global__ __launch_bounds__(kBlock) void k_core_process(
-    const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj, const uint32_t* q,
-    const uint32_t* qcount, uint32_t k, const uint32_t* __restrict__ lvl, uint32_t* deg,
-    uint32_t* qnext, uint32_t* qnext_count, uint32_t* heavy, uint32_t* heavy_count) {
-  const uint32_t cnt = *qcount;
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj, uint32_t* q0, uint32_t* q1, Ctl* ctl,
+    uint32_t k, const uint32_t* __restrict__ lvl, uint32_t* deg, uint32_t* heavy, uint32_t* heavy_count) {
+  if (ctl->action != kRoundPeel) return;
+  const uint32_t cur = ctl->cur, cnt = ctl->qcount[cur];
+  const uint32_t* q = cur ? q1 : q0;
+  uint32_t* qnext = cur ? q0 : q1;
+  uint32_t* qnext_count = &ctl->qcount[cur ^ 1u];
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   const int lane = lane64();
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < cnt; i += waves) {
@@ -537,11 +467,17 @@ __global__ void k_assign_rank(const uint64_t* keys, const uint32_t* vals, uint32
 
 // ---------------------------------------------- (b') induced subgraph ---
 
-// Induced subgraph of the alive vertices: sub[i] = alive vertex, new_id[v] = i.
+// Induced subgraph of the alive vertices: sub[i] = alive vertex, new_id[v] = i
+// (kNone for the removed vertices, so one lookup tells both).
 __global__ __launch_bounds__(kBlock) void k_alive_list(const uint32_t* __restrict__ lvl, uint32_t n,
                                                        uint32_t* list, uint32_t* new_id, uint32_t* count) {
   tile_compact(
-      n, [&](uint64_t v) { return lvl[v] == kNone; },
+      n,
+      [&](uint64_t v) {
+        const bool alive = lvl[v] == kNone;
+        if (!alive) new_id[v] = kNone;
+        return alive;
+      },
       [&](uint32_t at, uint64_t v) {
         list[at] = uint32_t(v);
         new_id[v] = at;
@@ -564,7 +500,6 @@ __global__ void k_sub_degrees(const uint32_t* __restrict__ list, const uint32_t*
 // from a per-row cursor.
 __global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ adj,
-                                                           const uint32_t* __restrict__ lvl,
                                                            const uint32_t* __restrict__ list,
                                                            const uint32_t* __restrict__ new_id,
                                                            const uint64_t* __restrict__ sub_off,
@@ -584,26 +519,21 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __res
     const uint64_t e = min(off[v + 1], b + kSlice);
     for (uint64_t j0 = b + (threadIdx.x & ~63u); j0 < e; j0 += blockDim.x) {
       const uint64_t j = j0 + lane;
-      uint32_t w = 0;
-      bool p = false;
-      if (j < e) {
-        w = adj[j];
-        p = lvl[w] == kNone;
-      }
+      const uint32_t nid = j < e ? new_id[adj[j]] : kNone;
+      const bool p = nid != kNone;
       const uint64_t mask = __ballot(p);
       if (!mask) continue;
       const int leader = __ffsll((unsigned long long)mask) - 1;
       unsigned long long at = 0;
       if (lane == leader) at = atomicAdd(&cursor[h], (unsigned long long)__popcll(mask));
       at = __shfl(at, leader);
-      if (p) sub_adj[sub_off[i] + at + uint64_t(__popcll(mask & ((1ull << lane) - 1ull)))] = new_id[w];
+      if (p) sub_adj[sub_off[i] + at + uint64_t(__popcll(mask & ((1ull << lane) - 1ull)))] = nid;
     }
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict__ off,
                                                      const uint32_t* __restrict__ adj,
-                                                     const uint32_t* __restrict__ lvl,
                                                      const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ new_id, uint32_t na,
                                                      const uint64_t* __restrict__ sub_off,
@@ -621,14 +551,10 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict_
     }
     for (uint64_t j0 = off[v]; j0 < e; j0 += kWave) {
       const uint64_t j = j0 + lane;
-      uint32_t w = 0;
-      bool p = false;
-      if (j < e) {
-        w = adj[j];
-        p = lvl[w] == kNone;
-      }
+      const uint32_t nid = j < e ? new_id[adj[j]] : kNone;
+      const bool p = nid != kNone;
       const uint64_t mask = __ballot(p);
-      if (p) sub_adj[pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = new_id[w];
+      if (p) sub_adj[pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = nid;
       pos += uint32_t(__popcll(mask));
     }
   }
@@ -847,64 +773,6 @@ __global__ __launch_bounds__(kBlock) void k_edge_frontier(const uint32_t* __rest
       [&](uint32_t at, uint64_t e) { q[at] = uint32_t(e); }, qcount);
 }
 
-__global__ void k_edge_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t r, uint32_t* state) {
-  const uint32_t cnt = *qcount;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
-    state[q[i]] = r;
-}
-
-__device__ __forceinline__ void peel_dec(uint32_t x, int32_t thr, int32_t* sup, uint32_t* qnext,
-                                         uint32_t* qnext_count) {
-  if (atomicSub(&sup[x], 1) == thr) qnext[atomicAdd(qnext_count, 1u)] = x;
-}
-
-// One sub-group per frontier edge e = (u, v). Triangles (u, v, w) whose other
-// two edges are not deleted lose e. Round-synchronous rule: a triangle with
-// two frontier edges is charged to the third edge once, by the frontier edge
-// of smaller id; a triangle with three frontier edges charges nobody. The
-// reference reaches the same sums with doubled supports and a `2 - #frontier`
-// decrement (src/kc.cu:122-146).
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_peel(
-    const uint32_t* __restrict__ off_u, const uint32_t* __restrict__ adj_u,
-    const uint32_t* __restrict__ eid_u, const uint32_t* __restrict__ src,
-    const uint32_t* __restrict__ dst, const uint32_t* __restrict__ q, const uint32_t* qcount,
-    uint32_t r, int32_t thr, const uint32_t* __restrict__ state, int32_t* sup, uint32_t* qnext,
-    uint32_t* qnext_count) {
-  const uint32_t cnt = *qcount;
-  const uint32_t groups = gridDim.x * (blockDim.x / G);
-  const uint32_t lane = threadIdx.x % G;
-  for (uint32_t qi = blockIdx.x * (blockDim.x / G) + threadIdx.x / G; qi < cnt; qi += groups) {
-    const uint32_t e = q[qi];
-    const uint32_t u = src[e], v = dst[e];
-    uint32_t ab = off_u[u], ae = off_u[u + 1];
-    uint32_t bb = off_u[v], be = off_u[v + 1];
-    if ((ae - ab) > (be - bb)) {
-      uint32_t t = ab; ab = bb; bb = t;
-      t = ae; ae = be; be = t;
-    }
-    for (uint32_t i0 = ab; i0 < ae; i0 += G) {
-      const uint32_t i = i0 + lane;
-      if (i >= ae) continue;
-      const uint32_t x = adj_u[i];
-      const uint32_t p = lower_bound_u32(adj_u, bb, be, x);
-      if (p >= be || adj_u[p] != x) continue;
-      const uint32_t ea = eid_u[i], eb = eid_u[p];
-      const uint32_t ma = state[ea], mb = state[eb];
-      if (ma < r || mb < r) continue;  // a side was deleted in an earlier round
-      const bool ina = (ma == r), inb = (mb == r);
-      if (!ina && !inb) {
-        peel_dec(ea, thr, sup, qnext, qnext_count);
-        peel_dec(eb, thr, sup, qnext, qnext_count);
-      } else if (ina && !inb) {
-        if (e < ea) peel_dec(eb, thr, sup, qnext, qnext_count);
-      } else if (!ina && inb) {
-        if (e < eb) peel_dec(ea, thr, sup, qnext, qnext_count);
-      }
-    }
-  }
-}
-
 __global__ __launch_bounds__(kBlock) void k_min_support(const uint32_t* __restrict__ state,
                                                         const int32_t* __restrict__ sup, uint32_t mc,
                                                         uint32_t* out) {
@@ -919,26 +787,6 @@ __global__ __launch_bounds__(kBlock) void k_min_support(const uint32_t* __restri
   if (threadIdx.x == 0) {
     for (int k = 1; k < kBlock / kWave; ++k) best = min(best, s_min[k]);
     if (best != kNone) atomicMin(out, best);
-  }
-}
-
-__global__ void k_alive_pairs(const uint32_t* __restrict__ state, uint32_t mc, uint32_t floor,
-                              const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
-                              const uint32_t* __restrict__ inv, uint2* out, uint32_t* count) {
-  for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < mc;
-       base += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t e = base + threadIdx.x;
-    bool p = false;
-    uint2 pr = make_uint2(0, 0);
-    if (e < mc) {
-      const uint32_t s = state[e];
-      p = (s == kNone) || (s > floor);
-      if (p) {
-        const uint32_t a = inv[src[e]], b = inv[dst[e]];
-        pr = make_uint2(min(a, b), max(a, b));
-      }
-    }
-    wave_append<uint2>(p, pr, out, count);
   }
 }
 
@@ -1041,70 +889,6 @@ __global__ __launch_bounds__(kBlock) void k_dense_support(const uint32_t* __rest
   }
 }
 
-__device__ __forceinline__ uint32_t dense_slot(const unsigned long long* __restrict__ bits0,
-                                               const uint32_t* __restrict__ pref0, uint64_t rowbase,
-                                               uint32_t w) {
-  const uint32_t i = w >> 6;
-  const unsigned long long below = bits0[rowbase + i] & ((1ull << (w & 63)) - 1ull);
-  return pref0[rowbase + i] + uint32_t(__popcll(below));
-}
-
-// Frontier processing on bitmaps: the common neighbours w of a frontier edge
-// (u, v) are the set bits of bits[u] & bits[v] (edges deleted in earlier
-// rounds are already cleared); the partner edge ids come from the slot of w
-// in the sorted rows. Same round-synchronous rule as k_peel.
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_dense_peel(
-    const uint32_t* __restrict__ q, const uint32_t* qcount, uint32_t r, int32_t thr,
-    const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
-    const unsigned long long* __restrict__ bits0, const uint32_t* __restrict__ pref0,
-    const unsigned long long* __restrict__ bits, uint32_t ws, const uint32_t* __restrict__ off_u,
-    const uint32_t* __restrict__ eid_u, const uint32_t* __restrict__ state, int32_t* sup,
-    uint32_t* qnext, uint32_t* qnext_count) {
-  const uint32_t cnt = *qcount;
-  const uint32_t groups = gridDim.x * (blockDim.x / G);
-  const uint32_t lane = threadIdx.x % G;
-  for (uint32_t qi = blockIdx.x * (blockDim.x / G) + threadIdx.x / G; qi < cnt; qi += groups) {
-    const uint32_t e = q[qi];
-    const uint32_t u = src[e], v = dst[e];
-    const uint64_t ru = uint64_t(u) * ws, rv = uint64_t(v) * ws;
-    const uint32_t ou = off_u[u], ov = off_u[v];
-    for (uint32_t i = lane; i < ws; i += G) {
-      unsigned long long t = bits[ru + i] & bits[rv + i];
-      while (t) {
-        const uint32_t w = (i << 6) + uint32_t(__ffsll(t) - 1);
-        t &= t - 1ull;
-        const uint32_t ea = eid_u[ou + dense_slot(bits0, pref0, ru, w)];
-        const uint32_t eb = eid_u[ov + dense_slot(bits0, pref0, rv, w)];
-        const uint32_t ma = state[ea], mb = state[eb];
-        if (ma < r || mb < r) continue;
-        const bool ina = (ma == r), inb = (mb == r);
-        if (!ina && !inb) {
-          peel_dec(ea, thr, sup, qnext, qnext_count);
-          peel_dec(eb, thr, sup, qnext, qnext_count);
-        } else if (ina && !inb) {
-          if (e < ea) peel_dec(eb, thr, sup, qnext, qnext_count);
-        } else if (!ina && inb) {
-          if (e < eb) peel_dec(ea, thr, sup, qnext, qnext_count);
-        }
-      }
-    }
-  }
-}
-
-// Clear the bits of the edges of a (stamped) frontier in the live bitmap.
-__global__ void k_dense_clear(const uint32_t* __restrict__ q, const uint32_t* qcount,
-                              const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
-                              unsigned long long* bits, uint32_t ws) {
-  const uint32_t cnt = *qcount;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-    const uint32_t e = q[i];
-    const uint32_t u = src[e], v = dst[e];
-    atomicAnd(&bits[uint64_t(u) * ws + (v >> 6)], ~(1ull << (v & 63)));
-    atomicAnd(&bits[uint64_t(v) * ws + (u >> 6)], ~(1ull << (u & 63)));
-  }
-}
-
 // ---------------------------------------------------- (d3) compaction ---
 
 __global__ __launch_bounds__(kBlock) void k_row_alive_count(const uint32_t* __restrict__ off_d,
@@ -1150,18 +934,6 @@ __global__ __launch_bounds__(kBlock) void k_row_compact(
 __global__ void k_count_din(const uint32_t* __restrict__ adj, uint32_t m, uint32_t* din) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
     atomicAdd(&din[adj[i]], 1u);
-}
-
-// Frontier stamp that also records, per base edge, the threshold of the
-// level that deleted it.
-__global__ void k_edge_stamp_thr(const uint32_t* q, const uint32_t* qcount, uint32_t r, uint32_t thr,
-                                 uint32_t* state, const uint32_t* __restrict__ orig, uint32_t* del_thr) {
-  const uint32_t cnt = *qcount;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-    const uint32_t e = q[i];
-    state[e] = r;
-    del_thr[orig[e]] = thr;
-  }
 }
 
 // Survivors of the level with threshold `floor` among the base edges.
@@ -1249,30 +1021,6 @@ void dense_support(const uint32_t* src, const uint32_t* dst, uint32_t mc, const 
     default: k_dense_support<64><<<grid_for(uint64_t(mc) * 64), kBlock, 0, s>>>(src, dst, mc, b, ws, state, sup); break;
   }
 }
-void dense_peel(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r, int32_t thr,
-                const uint32_t* src, const uint32_t* dst, const uint64_t* bits0, const uint32_t* pref0,
-                const uint64_t* bits, uint32_t ws, const uint32_t* off_u, const uint32_t* eid_u,
-                const uint32_t* state, int32_t* sup, uint32_t* qnext, uint32_t* qnext_count, hipStream_t s) {
-  if (!maxq) return;
-  auto b0 = reinterpret_cast<const unsigned long long*>(bits0);
-  auto b = reinterpret_cast<const unsigned long long*>(bits);
-#define KT_DENSE_PEEL(G)                                                                                    \
-  k_dense_peel<G><<<grid_for(uint64_t(maxq) * G), kBlock, 0, s>>>(q, qcount, r, thr, src, dst, b0, pref0, b, ws, \
-                                                                  off_u, eid_u, state, sup, qnext, qnext_count)
-  switch (dense_lanes(ws)) {
-    case 8: KT_DENSE_PEEL(8); break;
-    case 16: KT_DENSE_PEEL(16); break;
-    case 32: KT_DENSE_PEEL(32); break;
-    default: KT_DENSE_PEEL(64); break;
-  }
-#undef KT_DENSE_PEEL
-}
-void dense_clear(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, const uint32_t* src, const uint32_t* dst,
-                 uint64_t* bits, uint32_t ws, hipStream_t s) {
-  if (maxq)
-    k_dense_clear<<<grid_for(maxq), kBlock, 0, s>>>(q, qcount, src, dst, reinterpret_cast<unsigned long long*>(bits),
-                                                   ws);
-}
 void row_src(const uint32_t* off, uint32_t rows, uint32_t* src, hipStream_t s) {
   if (rows) k_row_src<<<grid_for(uint64_t(rows) * kWave), kBlock, 0, s>>>(off, rows, src);
 }
@@ -1292,10 +1040,6 @@ void row_compact(const uint32_t* off_d, const uint32_t* adj_d, const uint32_t* o
 }
 void count_in_degrees(const uint32_t* adj, uint32_t m, uint32_t* din, hipStream_t s) {
   if (m) k_count_din<<<grid_for(m), kBlock, 0, s>>>(adj, m, din);
-}
-void edge_stamp_thr(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r, uint32_t thr,
-                    uint32_t* state, const uint32_t* orig, uint32_t* del_thr, hipStream_t s) {
-  if (maxq) k_edge_stamp_thr<<<grid_for(maxq), kBlock, 0, s>>>(q, qcount, r, thr, state, orig, del_thr);
 }
 void truss_pairs(const uint32_t* del_thr, uint32_t m0, int32_t floor, const uint32_t* src0,
                  const uint32_t* dst0, const uint32_t* inv, uint32_t* out, uint32_t* count, hipStream_t s) {
@@ -1384,20 +1128,16 @@ void core_frontier_init(const uint32_t* lvl, const uint32_t* deg, uint32_t n, ui
                         uint32_t* qcount, hipStream_t s) {
   if (n) k_core_frontier<<<grid_tiles(n), kBlock, 0, s>>>(lvl, deg, n, k, q, qcount);
 }
-void core_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t k, uint32_t* lvl,
-                hipStream_t s) {
-  if (maxq) k_core_stamp<<<grid_for(maxq), kBlock, 0, s>>>(q, qcount, k, lvl);
-}
-void core_process(const uint64_t* off, const uint32_t* adj, const uint32_t* q, const uint32_t* qcount,
-                  uint32_t maxq, uint32_t k, uint32_t* lvl, uint32_t* deg, uint32_t* qnext,
-                  uint32_t* qnext_count, HeavyList hl, hipStream_t s) {
-  if (!maxq) return;
-  fill_u32(hl.count, 1, 0, s);
-  k_core_process<<<grid_for(uint64_t(maxq) * kWave), kBlock, 0, s>>>(off, adj, q, qcount, k, lvl, deg, qnext,
-                                                                     qnext_count, hl.items, hl.count);
-  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices);
-  k_core_process_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, hl.items, hl.count, hl.slices, k, lvl, deg, qnext,
-                                                     qnext_count);
+void core_round_start(Ctl* ctl, hipStream_t s) { k_vround_start<<<1, kWave, 0, s>>>(ctl); }
+void core_round(const uint64_t* off, const uint32_t* adj, uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t k,
+                uint32_t* lvl, uint32_t* deg, HeavyList hl, hipStream_t s) {
+  constexpr int kStampGrid = 512, kProcessGrid = 2048;
+  k_core_stamp<<<kStampGrid, kBlock, 0, s>>>(q0, q1, ctl, k, lvl);
+  k_core_process<<<kProcessGrid, kBlock, 0, s>>>(off, adj, q0, q1, ctl, k, lvl, deg, hl.items, hl.count);
+  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices, ctl);
+  k_core_process_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, hl.items, hl.count, hl.slices, k, lvl, deg, q0, q1,
+                                                     ctl);
+  k_vround_advance<<<1, kWave, 0, s>>>(ctl);
 }
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, uint32_t n,
                         uint32_t* deg, HeavyList hl, hipStream_t s) {
@@ -1405,7 +1145,7 @@ void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t
   fill_u32(hl.count, 1, 0, s);
   k_core_alive_deg<<<std::min(grid_for(uint64_t(n) * kWave), 8192), kBlock, 0, s>>>(off, adj, lvl, n, deg,
                                                                                     hl.items, hl.count);
-  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices);
+  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices, nullptr);
   k_core_alive_deg_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, lvl, hl.items, hl.count, hl.slices, deg);
 }
 void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s) {
@@ -1428,16 +1168,16 @@ void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_o
                  uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s) {
   if (na) k_sub_degrees<<<grid_for(na), kBlock, 0, s>>>(list, deg, to_orig, na, sub_deg, sub_orig);
 }
-void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, const uint32_t* list,
-              const uint32_t* new_id, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl,
-              uint64_t* cursor, uint32_t cursor_cap, hipStream_t s) {
+void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* new_id, uint32_t na,
+              const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl, uint64_t* cursor, uint32_t cursor_cap,
+              hipStream_t s) {
   if (!na) return;
   fill_u32(hl.count, 1, 0, s);
   fill_u64(cursor, cursor_cap, 0, s);
-  k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, lvl, list, new_id, na, sub_off, sub_adj,
+  k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, list, new_id, na, sub_off, sub_adj,
                                                                hl.items, hl.count);
-  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, list, hl.slices);
-  k_sub_fill_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, lvl, list, new_id, sub_off, sub_adj, hl.items, hl.count,
+  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, list, hl.slices, nullptr);
+  k_sub_fill_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, list, new_id, sub_off, sub_adj, hl.items, hl.count,
                                                  hl.slices, reinterpret_cast<unsigned long long*>(cursor));
 }
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to, const uint32_t* inv,
@@ -1502,43 +1242,8 @@ void edge_frontier_init(const uint32_t* state, const int32_t* sup, uint32_t mc, 
                         uint32_t* qcount, hipStream_t s) {
   if (mc) k_edge_frontier<<<grid_tiles(mc), kBlock, 0, s>>>(state, sup, mc, thr, q, qcount);
 }
-void edge_stamp(const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r, uint32_t* state,
-                hipStream_t s) {
-  if (maxq) k_edge_stamp<<<grid_for(maxq), kBlock, 0, s>>>(q, qcount, r, state);
-}
-void peel_process(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, const uint32_t* src,
-                  const uint32_t* dst, const uint32_t* q, const uint32_t* qcount, uint32_t maxq, uint32_t r,
-                  int32_t thr, int lanes, uint32_t* state, int32_t* sup, uint32_t* qnext,
-                  uint32_t* qnext_count, hipStream_t s) {
-  if (!maxq) return;
-  switch (lanes) {
-    case 8:
-      k_peel<8><<<grid_for(uint64_t(maxq) * 8), kBlock, 0, s>>>(off_u, adj_u, eid_u, src, dst, q, qcount, r,
-                                                                thr, state, sup, qnext, qnext_count);
-      break;
-    case 16:
-      k_peel<16><<<grid_for(uint64_t(maxq) * 16), kBlock, 0, s>>>(off_u, adj_u, eid_u, src, dst, q, qcount,
-                                                                  r, thr, state, sup, qnext, qnext_count);
-      break;
-    case 32:
-      k_peel<32><<<grid_for(uint64_t(maxq) * 32), kBlock, 0, s>>>(off_u, adj_u, eid_u, src, dst, q, qcount,
-                                                                  r, thr, state, sup, qnext, qnext_count);
-      break;
-    default:
-      k_peel<64><<<grid_for(uint64_t(maxq) * 64), kBlock, 0, s>>>(off_u, adj_u, eid_u, src, dst, q, qcount,
-                                                                  r, thr, state, sup, qnext, qnext_count);
-      break;
-  }
-}
 void edge_min_support(const uint32_t* state, const int32_t* sup, uint32_t mc, uint32_t* out, hipStream_t s) {
   if (mc) k_min_support<<<std::min(grid_for(mc), 2048), kBlock, 0, s>>>(state, sup, mc, out);
-}
-void edges_alive_above(const uint32_t* state, uint32_t mc, uint32_t stamp_floor, const uint32_t* src,
-                       const uint32_t* dst, const uint32_t* inv, uint32_t* pairs_out, uint32_t* count,
-                       hipStream_t s) {
-  if (mc)
-    k_alive_pairs<<<grid_for(mc), kBlock, 0, s>>>(state, mc, stamp_floor, src, dst, inv,
-                                                 reinterpret_cast<uint2*>(pairs_out), count);
 }
 
 }  // namespace kt

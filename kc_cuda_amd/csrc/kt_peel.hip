@@ -1,0 +1,428 @@
+// Device-driven edge-peel rounds (truss levels) on CDNA4.
+//
+// A level removes, round by round, every edge whose support is below the
+// threshold (reference: src/kc.cu:886-1056, __delete_edge / ktruss). Here a
+// round is three launches that read their frontier, its size and the round
+// stamp from the device control block:
+//   peel     walk the triangles of the frontier, decrement the partners,
+//            append edges that cross below the threshold to the next queue;
+//   advance  clear the frontier from the live bit matrix (dense cores) and, in
+//            the last workgroup, switch queues and decide the next round;
+//   stamp    stamp the next frontier (state, deletion threshold, frontier bits).
+// The host enqueues rounds in batches and reads the control block once per
+// batch; a round whose action is not "peel" does nothing.
+#include "kt_device.h"
+#include "kt_kernels.h"
+
+namespace kt {
+namespace {
+
+// Grids of the round kernels. Stamp and advance end with one ticket atomic
+// per workgroup, so they stay small; their loops stride over the frontier.
+constexpr int kStampGrid = 128;
+constexpr int kPeelGrid = 1024;
+constexpr int kAdvanceGrid = 64;
+constexpr int kClearGrid = 512;
+
+__device__ __forceinline__ uint32_t decide_round(uint32_t c, unsigned long long alive, const RoundRule& rule,
+                                                 bool allow_compact) {
+  if (c == 0) return kRoundIdle;
+  if (allow_compact && 2ull * alive <= rule.mc) return kRoundCompact;
+  if (c == alive) return kRoundEmpty;
+  const bool drop = rule.dense ? double(c) * (double(rule.ws) + rule.tri_cost * double(rule.thr)) >=
+                                     double(alive) * double(rule.ws)
+                               : double(c) * rule.drop_ratio >= double(alive);
+  return drop ? kRoundDrop : kRoundPeel;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void k_round_start(Ctl* ctl, uint32_t r, unsigned long long alive, RoundRule rule, uint32_t reset) {
+  if (threadIdx.x != 0) return;
+  const uint32_t c = ctl->qcount[0];
+  ctl->cur = 0;
+  ctl->qcount[1] = 0;
+  ctl->round = r;
+  ctl->alive = alive;
+  ctl->ticket = 0;
+  ctl->stamped = 0;
+  if (reset) ctl->rounds_done = 0;
+  ctl->action = decide_round(c, alive, rule, false);
+}
+
+// Stamp the current frontier once: state = round, deletion threshold of its
+// base edge = thr, and (walked frontiers of dense cores) its frontier bits.
+// The rounds queued after a drop or an empty level find it stamped already.
+__global__ __launch_bounds__(kBlock) void k_round_stamp(uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t thr,
+                                                        uint32_t* state, const uint32_t* __restrict__ orig,
+                                                        uint32_t* del_thr, const uint32_t* __restrict__ src,
+                                                        const uint32_t* __restrict__ dst,
+                                                        unsigned long long* fbits, uint32_t ws) {
+  const uint32_t a = ctl->action;
+  if (ctl->stamped || (a != kRoundPeel && a != kRoundDrop && a != kRoundEmpty)) return;
+  const uint32_t cur = ctl->cur, cnt = ctl->qcount[cur], r = ctl->round;
+  const uint32_t* q = cur ? q1 : q0;
+  const bool fb = fbits && a == kRoundPeel;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const uint32_t e = q[i];
+    state[e] = r;
+    del_thr[orig[e]] = thr;
+    if (fb) {
+      const uint32_t u = src[e], v = dst[e];
+      atomicOr(&fbits[uint64_t(u) * ws + (v >> 6)], 1ull << (v & 63));
+      atomicOr(&fbits[uint64_t(v) * ws + (u >> 6)], 1ull << (u & 63));
+    }
+  }
+  if (last_block(&ctl->ticket) && threadIdx.x == 0) {
+    ctl->alive -= cnt;
+    ctl->rounds_done += 1;
+    ctl->stamped = 1;
+  }
+}
+
+__device__ __forceinline__ void clear_edge_bits(uint32_t u, uint32_t v, unsigned long long* bits,
+                                                unsigned long long* fbits, uint32_t ws) {
+  const uint64_t a = uint64_t(u) * ws + (v >> 6), b = uint64_t(v) * ws + (u >> 6);
+  const unsigned long long ma = ~(1ull << (v & 63)), mb = ~(1ull << (u & 63));
+  atomicAnd(&bits[a], ma);
+  atomicAnd(&bits[b], mb);
+  if (fbits) {
+    atomicAnd(&fbits[a], ma);
+    atomicAnd(&fbits[b], mb);
+  }
+}
+
+// Close a walked round: clear its frontier from the bit matrices, then (last
+// workgroup) make the next queue current and decide what the next round does.
+__global__ __launch_bounds__(kBlock) void k_round_advance(uint32_t* q0, uint32_t* q1, Ctl* ctl,
+                                                          const uint32_t* __restrict__ src,
+                                                          const uint32_t* __restrict__ dst,
+                                                          unsigned long long* bits, unsigned long long* fbits,
+                                                          uint32_t ws, RoundRule rule) {
+  if (ctl->action != kRoundPeel || !ctl->stamped) return;
+  const uint32_t cur = ctl->cur;
+  if (bits) {
+    const uint32_t cnt = ctl->qcount[cur];
+    const uint32_t* q = cur ? q1 : q0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+      const uint32_t e = q[i];
+      clear_edge_bits(src[e], dst[e], bits, fbits, ws);
+    }
+  }
+  if (last_block(&ctl->ticket) && threadIdx.x == 0) {
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t c = ctl->qcount[nxt];
+    ctl->qcount[cur] = 0;
+    ctl->cur = nxt;
+    ctl->round += 1;
+    ctl->stamped = 0;
+    ctl->action = decide_round(c, ctl->alive, rule, !rule.dense);
+  }
+}
+
+// Clear the current frontier from the bit matrices (a dropped frontier).
+__global__ __launch_bounds__(kBlock) void k_frontier_clear(const uint32_t* q0, const uint32_t* q1, const Ctl* ctl,
+                                                           const uint32_t* __restrict__ src,
+                                                           const uint32_t* __restrict__ dst,
+                                                           unsigned long long* bits, unsigned long long* fbits,
+                                                           uint32_t ws) {
+  const uint32_t cur = ctl->cur, cnt = ctl->qcount[cur];
+  const uint32_t* q = cur ? q1 : q0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const uint32_t e = q[i];
+    clear_edge_bits(src[e], dst[e], bits, fbits, ws);
+  }
+}
+
+__device__ __forceinline__ bool dec_crosses(uint32_t x, int32_t thr, int32_t* sup) {
+  return x != kNone && atomicSub(&sup[x], 1) == thr;
+}
+
+// Walked round on a bit-matrix core. One wave per (frontier edge, 64-word
+// chunk of its rows): the common neighbours w of e = (u, v) in the chunk are
+// the set bits of live[u] & live[v]; their count is scanned across the wave
+// and the triangles are dealt out evenly, one per lane per step, whatever
+// words they sit in. Frontier bits say at word level whether (u, w) and
+// (v, w) are in the frontier, so a triangle costs only the edge-id lookups
+// of the partners it decrements. Round-synchronous rule: a triangle with one
+// frontier edge charges both partners; with two, the third edge is charged
+// once -- by the frontier edge whose non-shared endpoint is larger than the
+// apex (z < x for the third edge (x, z)); with three, nobody. The supports
+// after the round equal the reference's `2 - #frontier` decrements on
+// doubled counts (src/kc.cu:122-146).
+template <bool kEidm>
+__global__ __launch_bounds__(kBlock) void k_dense_peel_round(
+    uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, const uint32_t* __restrict__ src,
+    const uint32_t* __restrict__ dst, const unsigned long long* __restrict__ bits,
+    const unsigned long long* __restrict__ fbits, uint32_t ws, const uint32_t* __restrict__ eidm, uint32_t nc,
+    const unsigned long long* __restrict__ bits0, const uint32_t* __restrict__ pref0,
+    const uint32_t* __restrict__ off_u, const uint32_t* __restrict__ eid_u, int32_t* sup) {
+  if (ctl->action != kRoundPeel || !ctl->stamped) return;
+  const uint32_t cur = ctl->cur;
+  const uint32_t cnt = ctl->qcount[cur];
+  const uint32_t* q = cur ? q1 : q0;
+  uint32_t* qn = cur ? q0 : q1;
+  uint32_t* qn_count = &ctl->qcount[cur ^ 1u];
+  __shared__ unsigned long long s_t[kBlock], s_fa[kBlock], s_fb[kBlock];
+  __shared__ uint32_t s_inc[kBlock];
+  const int lane = lane64();
+  const uint32_t wbase = threadIdx.x & ~uint32_t(kWave - 1);
+  const uint32_t nch = (ws + kWave - 1) / kWave;
+  const uint64_t items = uint64_t(cnt) * nch;
+  const uint64_t nw = uint64_t(gridDim.x) * (kBlock / kWave);
+  auto eid_of = [&](uint32_t a, uint32_t w) -> uint32_t {
+    if (kEidm) return eidm[uint64_t(a) * nc + w];
+    const uint64_t ra = uint64_t(a) * ws;
+    const uint32_t i = w >> 6;
+    const unsigned long long below = bits0[ra + i] & ((1ull << (w & 63)) - 1ull);
+    return eid_u[off_u[a] + pref0[ra + i] + uint32_t(__popcll(below))];
+  };
+  for (uint64_t it = uint64_t(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; it < items; it += nw) {
+    const uint32_t qi = uint32_t(it / nch), ch = uint32_t(it - uint64_t(qi) * nch);
+    const uint32_t e = q[qi];
+    const uint32_t u = src[e], v = dst[e];
+    const uint64_t ru = uint64_t(u) * ws, rv = uint64_t(v) * ws;
+    const uint32_t i = ch * kWave + uint32_t(lane);
+    unsigned long long t = 0, fa = 0, fb = 0;
+    if (i < ws) {
+      t = bits[ru + i] & bits[rv + i];
+      fa = fbits[ru + i] & t;
+      fb = fbits[rv + i] & t;
+      t &= ~(fa & fb);  // both partners in the frontier: nobody is charged
+    }
+    uint32_t inc = uint32_t(__popcll(t));
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o);
+      if (lane >= o) inc += y;
+    }
+    const uint32_t total = __shfl(inc, kWave - 1);
+    s_t[threadIdx.x] = t;
+    s_fa[threadIdx.x] = fa;
+    s_fb[threadIdx.x] = fb;
+    s_inc[threadIdx.x] = inc;
+    wave_sync();
+    for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
+      const uint32_t j = j0 + uint32_t(lane);
+      uint32_t xa = kNone, xb = kNone;  // partner edges (u, w) and (v, w) to decrement
+      if (j < total) {
+        uint32_t lo = 0, hi = kWave - 1;  // owner: first lane whose inclusive count exceeds j
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_inc[wbase + mid] > j)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        const unsigned long long tw = s_t[wbase + lo];
+        const uint32_t b = select_bit(tw, j - (s_inc[wbase + lo] - uint32_t(__popcll(tw))));
+        const uint32_t w = ((ch * kWave + lo) << 6) + b;
+        const bool ina = (s_fa[wbase + lo] >> b) & 1ull, inb = (s_fb[wbase + lo] >> b) & 1ull;
+        if (!ina && !inb) {
+          xa = eid_of(u, w);
+          xb = eid_of(v, w);
+        } else if (ina) {  // (u, w) in the frontier, third edge (v, w)
+          if (w < v) xb = eid_of(v, w);
+        } else {  // (v, w) in the frontier, third edge (u, w)
+          if (w < u) xa = eid_of(u, w);
+        }
+      }
+      const bool pa = dec_crosses(xa, thr, sup);
+      const bool pb = dec_crosses(xb, thr, sup);
+      wave_append(pa, xa, qn, qn_count);
+      wave_append(pb, xb, qn, qn_count);
+    }
+    wave_sync();
+  }
+}
+
+// Walked round on a sorted-row core: G lanes per frontier edge stream the
+// shorter undirected row and binary-search the longer one; partners' states
+// say whether they are in this round's frontier (== round) or gone (< round).
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_sparse_peel_round(
+    uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, const uint32_t* __restrict__ off_u,
+    const uint32_t* __restrict__ adj_u, const uint32_t* __restrict__ eid_u, const uint32_t* __restrict__ src,
+    const uint32_t* __restrict__ dst, const uint32_t* __restrict__ state, int32_t* sup) {
+  if (ctl->action != kRoundPeel || !ctl->stamped) return;
+  const uint32_t cur = ctl->cur;
+  const uint32_t cnt = ctl->qcount[cur], r = ctl->round;
+  const uint32_t* q = cur ? q1 : q0;
+  uint32_t* qn = cur ? q0 : q1;
+  uint32_t* qn_count = &ctl->qcount[cur ^ 1u];
+  const uint32_t groups = gridDim.x * (blockDim.x / G);
+  const uint32_t lane = threadIdx.x % G;
+  // every lane of a wave runs the same number of steps so queue appends can
+  // be wave-aggregated
+  const uint32_t steps = (cnt + groups - 1) / groups;
+  for (uint32_t s = 0; s < steps; ++s) {
+    const uint32_t qi = s * groups + blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+    uint32_t ab = 0, ae = 0, bb = 0, be = 0, e = 0;
+    if (qi < cnt) {
+      e = q[qi];
+      const uint32_t u = src[e], v = dst[e];
+      ab = off_u[u], ae = off_u[u + 1];
+      bb = off_u[v], be = off_u[v + 1];
+      if ((ae - ab) > (be - bb)) {
+        uint32_t t = ab; ab = bb; bb = t;
+        t = ae; ae = be; be = t;
+      }
+    }
+    // longest shorter-row of the wave bounds the inner loop
+    uint32_t len = ae - ab;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) len = max(len, uint32_t(__shfl_xor(len, o)));
+    for (uint32_t i0 = 0; i0 < len; i0 += G) {
+      const uint32_t i = ab + i0 + lane;
+      uint32_t xa = kNone, xb = kNone;
+      if (i < ae) {
+        const uint32_t x = adj_u[i];
+        const uint32_t p = lower_bound_u32(adj_u, bb, be, x);
+        if (p < be && adj_u[p] == x) {
+          const uint32_t ea = eid_u[i], eb = eid_u[p];
+          const uint32_t ma = state[ea], mb = state[eb];
+          if (ma >= r && mb >= r) {  // neither side deleted in an earlier round
+            const bool ina = (ma == r), inb = (mb == r);
+            if (!ina && !inb) {
+              xa = ea;
+              xb = eb;
+            } else if (ina && !inb) {
+              if (e < ea) xb = eb;
+            } else if (!ina && inb) {
+              if (e < eb) xa = ea;
+            }
+          }
+        }
+      }
+      const bool pa = dec_crosses(xa, thr, sup);
+      const bool pb = dec_crosses(xb, thr, sup);
+      wave_append(pa, xa, qn, qn_count);
+      wave_append(pb, xb, qn, qn_count);
+    }
+  }
+}
+
+// Edge-id matrix of a bit-matrix core: eidm[u * nc + w] = id of edge {u, w}
+// (entries of non-edges are never read).
+__global__ __launch_bounds__(kBlock) void k_dense_eidm(const uint32_t* __restrict__ off_u,
+                                                       const uint32_t* __restrict__ adj_u,
+                                                       const uint32_t* __restrict__ eid_u, uint32_t nc,
+                                                       uint32_t* eidm) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < nc; u += waves)
+    for (uint32_t j = off_u[u] + lane; j < off_u[u + 1]; j += kWave) eidm[uint64_t(u) * nc + adj_u[j]] = eid_u[j];
+}
+
+// flag[u] = 1 for every endpoint of a live edge (plain stores of one value).
+__global__ void k_live_vertex_flags(const uint32_t* __restrict__ state, const uint32_t* __restrict__ src,
+                                    const uint32_t* __restrict__ dst, uint32_t mc, uint32_t* flag) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < mc; e += gridDim.x * blockDim.x)
+    if (state[e] == kNone) {
+      flag[src[e]] = 1u;
+      flag[dst[e]] = 1u;
+    }
+}
+
+// Renumber the endpoints of compacted edges through the (monotone) map.
+__global__ void k_relabel_edges(uint32_t* adj, uint32_t* src, uint32_t m, const uint32_t* __restrict__ vmap) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    adj[i] = vmap[adj[i]];
+    src[i] = vmap[src[i]];
+  }
+}
+
+// Row offsets of the renumbered rows: vertices without a live edge have empty
+// rows, so the kept rows stay contiguous.
+__global__ void k_relabel_offsets(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ vmap, uint32_t n,
+                                  uint32_t* off) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u <= n; u += gridDim.x * blockDim.x)
+    if (u == n || vmap[u + 1] != vmap[u]) off[vmap[u]] = roff[u];
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- wrappers ---
+
+void live_vertex_flags(const uint32_t* state, const uint32_t* src, const uint32_t* dst, uint32_t mc, uint32_t* flag,
+                       hipStream_t s) {
+  if (mc) k_live_vertex_flags<<<grid_for(mc), kBlock, 0, s>>>(state, src, dst, mc, flag);
+}
+void relabel_edges(uint32_t* adj, uint32_t* src, uint32_t m, const uint32_t* vmap, hipStream_t s) {
+  if (m) k_relabel_edges<<<grid_for(m), kBlock, 0, s>>>(adj, src, m, vmap);
+}
+void relabel_offsets(const uint32_t* roff, const uint32_t* vmap, uint32_t n, uint32_t* off, hipStream_t s) {
+  k_relabel_offsets<<<grid_for(uint64_t(n) + 1), kBlock, 0, s>>>(roff, vmap, n, off);
+}
+
+void round_start(Ctl* ctl, uint32_t r, uint64_t alive, const RoundRule& rule, bool reset, hipStream_t s) {
+  k_round_start<<<1, kWave, 0, s>>>(ctl, r, alive, rule, reset ? 1u : 0u);
+}
+
+void round_stamp(uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t thr, uint32_t* state, const uint32_t* orig,
+                 uint32_t* del_thr, const uint32_t* src, const uint32_t* dst, uint64_t* fbits, uint32_t ws,
+                 hipStream_t s) {
+  k_round_stamp<<<kStampGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, state, orig, del_thr, src, dst,
+                                              reinterpret_cast<unsigned long long*>(fbits), ws);
+}
+
+void round_advance(uint32_t* q0, uint32_t* q1, Ctl* ctl, const uint32_t* src, const uint32_t* dst, uint64_t* bits,
+                   uint64_t* fbits, uint32_t ws, const RoundRule& rule, hipStream_t s) {
+  k_round_advance<<<bits ? kAdvanceGrid : 1, kBlock, 0, s>>>(q0, q1, ctl, src, dst,
+                                                            reinterpret_cast<unsigned long long*>(bits),
+                                                            reinterpret_cast<unsigned long long*>(fbits), ws, rule);
+}
+
+void frontier_clear(const uint32_t* q0, const uint32_t* q1, const Ctl* ctl, const uint32_t* src, const uint32_t* dst,
+                    uint64_t* bits, uint64_t* fbits, uint32_t ws, hipStream_t s) {
+  k_frontier_clear<<<kClearGrid, kBlock, 0, s>>>(q0, q1, ctl, src, dst, reinterpret_cast<unsigned long long*>(bits),
+                                                 reinterpret_cast<unsigned long long*>(fbits), ws);
+}
+
+void dense_peel_round(uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, const uint32_t* src, const uint32_t* dst,
+                      const uint64_t* bits, const uint64_t* fbits, uint32_t ws, const uint32_t* eidm, uint32_t nc,
+                      const uint64_t* bits0, const uint32_t* pref0, const uint32_t* off_u, const uint32_t* eid_u,
+                      int32_t* sup, hipStream_t s) {
+  auto b = reinterpret_cast<const unsigned long long*>(bits);
+  auto f = reinterpret_cast<const unsigned long long*>(fbits);
+  auto b0 = reinterpret_cast<const unsigned long long*>(bits0);
+  if (eidm)
+    k_dense_peel_round<true><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, src, dst, b, f, ws, eidm, nc, b0, pref0,
+                                                          off_u, eid_u, sup);
+  else
+    k_dense_peel_round<false><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, src, dst, b, f, ws, eidm, nc, b0, pref0,
+                                                           off_u, eid_u, sup);
+}
+
+void sparse_peel_round(uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, int lanes, const uint32_t* off_u,
+                       const uint32_t* adj_u, const uint32_t* eid_u, const uint32_t* src, const uint32_t* dst,
+                       const uint32_t* state, int32_t* sup, hipStream_t s) {
+  switch (lanes) {
+    case 8:
+      k_sparse_peel_round<8><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, off_u, adj_u, eid_u, src, dst, state, sup);
+      break;
+    case 16:
+      k_sparse_peel_round<16><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, off_u, adj_u, eid_u, src, dst, state,
+                                                           sup);
+      break;
+    case 32:
+      k_sparse_peel_round<32><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, off_u, adj_u, eid_u, src, dst, state,
+                                                           sup);
+      break;
+    default:
+      k_sparse_peel_round<64><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, off_u, adj_u, eid_u, src, dst, state,
+                                                           sup);
+      break;
+  }
+}
+
+void dense_eidm(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, uint32_t nc, uint32_t* eidm,
+                hipStream_t s) {
+  if (nc) k_dense_eidm<<<grid_for(uint64_t(nc) * kWave), kBlock, 0, s>>>(off_u, adj_u, eid_u, nc, eidm);
+}
+
+}  // namespace kt

@@ -41,6 +41,7 @@ struct LevelGraph {
 // current edge back to its base id, where its deletion threshold is recorded.
 struct CoreView {
   uint32_t nc = 0, mc = 0, max_dout = 0;
+  uint32_t nc0 = 0;  // vertices of the base set (rank ids index inv)
   uint32_t *inv = nullptr, *off_d = nullptr, *adj_d = nullptr, *src = nullptr, *orig = nullptr;
   uint32_t *off_u = nullptr, *adj_u = nullptr, *eid_u = nullptr, *state = nullptr;
   int32_t* sup = nullptr;
@@ -48,11 +49,14 @@ struct CoreView {
   uint32_t mc0 = 0, max_dout0 = 0;
   uint32_t *off_d0 = nullptr, *adj_d0 = nullptr, *src0 = nullptr, *del_thr = nullptr;
   int work = -1;  // which work set holds the current edges (-1: base)
-  // dense cores: bit-matrix rows (ws words each) of all / live edges
+  // dense cores: bit-matrix rows (ws words each) of all / live / frontier
+  // edges, per-word prefix popcounts of the static rows, and (when nc^2
+  // entries fit the budget) the edge id of every (u, w) pair
   bool dense = false;
   uint32_t ws = 0;
-  uint64_t *bits0 = nullptr, *bits = nullptr;
+  uint64_t *bits0 = nullptr, *bits = nullptr, *fbits = nullptr;
   uint32_t* pref0 = nullptr;
+  uint32_t* eidm = nullptr;
 };
 
 struct TrussOutcome {
@@ -105,8 +109,13 @@ class Solver {
   void build_undirected_view(CoreView* cv);
   void count_support(CoreView* cv, float* ms);
   // Keep only the live edges (state == kNone) of the current view; with
-  // carry_sup their supports are kept, otherwise recounted exactly.
-  void compact_core(CoreView* cv, bool carry_sup);
+  // carry_sup their supports are kept, otherwise recounted exactly. With a
+  // vertex map (exclusive scan of live-vertex flags, nc2 kept vertices) the
+  // vertices are renumbered too and a dense view is rebuilt at its new size.
+  void compact_core(CoreView* cv, bool carry_sup, const uint32_t* vmap = nullptr, uint32_t nc2 = 0);
+  // Dense views: after a dropped frontier, rebuild on the vertices that still
+  // have live edges when that shrinks the rows enough; recounts supports.
+  bool try_redensify(CoreView* cv);
   // exact_early: stop at a first level of early_ans edges and report early_k
   // whatever the minimum support (the reference's rule, src/kc.cu:1240).
   TrussOutcome truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k,
@@ -137,6 +146,11 @@ class Solver {
   void solve_csr(kt_result* res);
   uint32_t read_u32(const uint32_t* dptr);
   uint64_t read_u64(const uint64_t* dptr);
+  Ctl read_ctl();
+  RoundRule round_rule(const CoreView& cv, int32_t thr) const;
+  // frontier of the live edges below thr, stamped; the rounds' starting state
+  void start_rounds(CoreView* cv, int32_t thr, uint32_t r, uint64_t alive, bool reset);
+  void enqueue_rounds(CoreView* cv, int32_t thr, int count);
   void zero_u32(uint32_t* dptr);
   void exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n_plus_1);
   void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n_plus_1);
@@ -153,11 +167,11 @@ class Solver {
   bool force_edge_count_ = false;
   bool force_atomic_build_ = false;
   int core_mode_ = 0;
-  DevBuf bits0_, bits_, pref0_, heavy_;
+  DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_;
+  uint64_t eidm_budget_ = uint64_t(4) << 30;  // bytes for the edge-id matrix
   double dense_tri_cost_ = 8.0, sparse_drop_ratio_ = 3.0;
   bool trace_ = false;  // KT_TRACE=1: per-round frontier sizes on stderr
   void make_dense(CoreView* cv);
-  bool drop_frontier(const CoreView& cv, uint64_t c, uint64_t alive, int32_t thr) const;
   HeavyList heavy() {
     heavy_.ensure(sizeof(uint32_t) * 2 * (size_t(n_) + 2));
     return HeavyList{heavy_.as<uint32_t>(), &ctl_->pad[1], heavy_.as<uint32_t>() + size_t(n_) + 2};

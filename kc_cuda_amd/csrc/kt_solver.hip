@@ -39,13 +39,14 @@ Solver::Solver(int device) : device_(device) {
   KT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   KT_HIP(hipMalloc(&ctl_, sizeof(Ctl)));
   KT_HIP(hipMemsetAsync(ctl_, 0, sizeof(Ctl), stream_));
-  KT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 64, hipHostMallocDefault));
+  KT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 256, hipHostMallocDefault));
   for (auto& e : ev_) KT_HIP(hipEventCreate(&e));
   KT_HIP(hipStreamSynchronize(stream_));
   // cost-model knobs of the frontier-versus-recount choice (tuning only)
   if (const char* v = std::getenv("KT_DENSE_TRI_COST")) dense_tri_cost_ = std::atof(v);
   if (const char* v = std::getenv("KT_SPARSE_DROP_RATIO")) sparse_drop_ratio_ = std::atof(v);
   if (const char* v = std::getenv("KT_TRACE")) trace_ = v[0] == '1';
+  if (const char* v = std::getenv("KT_EIDM_BUDGET_MB")) eidm_budget_ = uint64_t(std::atoll(v)) << 20;
 }
 
 Solver::~Solver() {
@@ -55,7 +56,8 @@ Solver::~Solver() {
                    &keys_, &keys2_, &vals_, &vals2_, &to_, &inv_, &inv_orig_, &dout_, &off_d_,
                    &adj_d_, &src_, &din_, &din_off_, &tkeys_, &tvals_, &tiota_, &in_src_, &off_u_,
                    &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &del_thr_, &orig0_,
-                   &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &pref0_, &heavy_};
+                   &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &fbits_, &pref0_, &eidm_, &heavy_,
+                   &vflag_, &vmap_, &roff_};
   for (auto* b : all) b->release();
   for (auto& w : work_) {
     w.off.release();
@@ -89,6 +91,15 @@ uint64_t Solver::read_u64(const uint64_t* dptr) {
   uint64_t v;
   std::memcpy(&v, pinned_, sizeof(v));
   return v;
+}
+
+Ctl Solver::read_ctl() {
+  static_assert(sizeof(Ctl) <= 256, "control block exceeds the pinned readback buffer");
+  KT_HIP(hipMemcpyAsync(pinned_, ctl_, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  Ctl c;
+  std::memcpy(&c, pinned_, sizeof(c));
+  return c;
 }
 
 void Solver::zero_u32(uint32_t* dptr) { fill_u32(dptr, 1, 0, stream_); }

@@ -39,24 +39,34 @@ constexpr uint32_t kMinCut = 256;  // smaller alive sets stay on their level
 
 // Removes every alive vertex of degree < k, round by round; returns the
 // number removed. lvl[] of each removed vertex becomes k.
+// Rounds run on the device in batches; the host reads the control block
+// between batches only.
 uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
-  uint32_t* qc = ctl_->qcount;
-  uint32_t* q[2] = {q0_.as<uint32_t>(), q1_.as<uint32_t>()};
-  zero_u32(&qc[0]);
-  core_frontier_init(g.lvl, g.deg, g.n, k, q[0], &qc[0], stream_);
-  uint32_t c = read_u32(&qc[0]);
-  uint64_t removed = 0;
-  int cur = 0;
-  while (c) {
-    if (trace_) std::fprintf(stderr, "[kt] vertex-peel level_n=%u k=%u frontier=%u\n", g.n, k, c);
-    core_stamp(q[cur], &qc[cur], c, k, g.lvl, stream_);
-    zero_u32(&qc[cur ^ 1]);
-    core_process(g.off, g.adj, q[cur], &qc[cur], c, k, g.lvl, g.deg, q[cur ^ 1], &qc[cur ^ 1], heavy(), stream_);
-    removed += c;
-    cur ^= 1;
-    c = read_u32(&qc[cur]);
+  constexpr int kMaxBatch = 32;
+  q0_.ensure(sizeof(uint32_t) * (size_t(g.n) + 1));
+  q1_.ensure(sizeof(uint32_t) * (size_t(g.n) + 1));
+  uint32_t* q0 = q0_.as<uint32_t>();
+  uint32_t* q1 = q1_.as<uint32_t>();
+  zero_u32(&ctl_->qcount[0]);
+  core_frontier_init(g.lvl, g.deg, g.n, k, q0, &ctl_->qcount[0], stream_);
+  core_round_start(ctl_, stream_);
+  const HeavyList hl = heavy();
+  int batch = 2;
+  uint64_t queued = 0;  // every round with a frontier removes a vertex
+  for (;;) {
+    for (int i = 0; i < batch; ++i) core_round(g.off, g.adj, q0, q1, ctl_, k, g.lvl, g.deg, hl, stream_);
+    queued += batch;
+    const Ctl h = read_ctl();
+    if (trace_)
+      std::fprintf(stderr, "[kt] vertex-peel level_n=%u k=%u batch=%d removed=%llu frontier=%u\n", g.n, k, batch,
+                   h.removed, h.action == kRoundPeel ? h.qcount[h.cur] : 0u);
+    if (h.action != kRoundPeel) {
+      if (h.removed > g.n) fail("vertex peel bookkeeping mismatch");
+      return h.removed;
+    }
+    if (queued > uint64_t(g.n) + 2 * kMaxBatch) fail("vertex peel did not converge");
+    batch = std::min(batch * 2, kMaxBatch);
   }
-  return removed;
 }
 
 // Induced subgraph of the alive vertices of `p` as the next level.
@@ -88,7 +98,7 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
   s.adj = s.b_adj.as<uint32_t>();
   const uint32_t cap = uint32_t(std::min<uint64_t>(na, p.slots / 8192 + 1));
   cursor_.ensure(sizeof(uint64_t) * (size_t(cap) + 1));
-  sub_fill(p.off, p.adj, p.lvl, list_.as<uint32_t>(), new_id_.as<uint32_t>(), na, s.off, s.adj, heavy(),
+  sub_fill(p.off, p.adj, list_.as<uint32_t>(), new_id_.as<uint32_t>(), na, s.off, s.adj, heavy(),
            cursor_.as<uint64_t>(), cap, stream_);
   fill_u32(s.lvl, na, kNone, stream_);
   ++depth_;
@@ -171,7 +181,7 @@ void Solver::extract_core(LevelGraph& g, CoreView* cv) {
   core_collect(g.lvl, g.deg, g.to_orig, n, keys_.as<uint64_t>(), vals_.as<uint32_t>(), &ctl_->nalive, stream_);
   const uint32_t nc = read_u32(&ctl_->nalive);
   *cv = CoreView{};
-  cv->nc = nc;
+  cv->nc = cv->nc0 = nc;
   inv_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
   inv_orig_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
   dout_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
@@ -238,13 +248,23 @@ void Solver::make_dense(CoreView* cv) {
   const size_t words = size_t(cv->nc) * ws;
   bits0_.ensure(sizeof(uint64_t) * std::max<size_t>(words, 1));
   bits_.ensure(sizeof(uint64_t) * std::max<size_t>(words, 1));
+  fbits_.ensure(sizeof(uint64_t) * std::max<size_t>(words, 1));
   pref0_.ensure(sizeof(uint32_t) * std::max<size_t>(words, 1));
   cv->dense = true;
   cv->ws = ws;
   cv->bits0 = bits0_.as<uint64_t>();
   cv->bits = bits_.as<uint64_t>();
+  cv->fbits = fbits_.as<uint64_t>();
   cv->pref0 = pref0_.as<uint32_t>();
   dense_rows(cv->off_u, cv->adj_u, cv->nc, ws, cv->bits0, cv->bits, cv->pref0, stream_);
+  fill_u64(cv->fbits, words, 0, stream_);
+  cv->eidm = nullptr;
+  const uint64_t eb = sizeof(uint32_t) * uint64_t(cv->nc) * cv->nc;
+  if (eb && eb <= eidm_budget_) {
+    eidm_.ensure(eb);
+    cv->eidm = eidm_.as<uint32_t>();
+    dense_eidm(cv->off_u, cv->adj_u, cv->eid_u, cv->nc, cv->eidm, stream_);
+  }
 }
 
 // Undirected CSR of the current oriented edges: row w = [sources of w's
@@ -306,7 +326,7 @@ void Solver::count_support(CoreView* cv, float* ms) {
 
 // ----------------------------------------------------- (d3) compaction ---
 
-void Solver::compact_core(CoreView* cv, bool carry_sup) {
+void Solver::compact_core(CoreView* cv, bool carry_sup, const uint32_t* vmap, uint32_t nc2) {
   const int t = cv->work == 0 ? 1 : 0;
   EdgeSet& w = work_[t];
   const uint32_t nc = cv->nc;
@@ -314,17 +334,27 @@ void Solver::compact_core(CoreView* cv, bool carry_sup) {
   fill_u32(dout_.as<uint32_t>(), uint64_t(nc) + 1, 0, stream_);
   row_alive_count(cv->off_d, cv->state, nc, dout_.as<uint32_t>(), stream_);
   w.off.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
-  exclusive_scan_u32(dout_.as<uint32_t>(), w.off.as<uint32_t>(), nc + 1);
-  const uint32_t mc2 = read_u32(w.off.as<uint32_t>() + nc);
+  // offsets of the kept edges in the old row numbering
+  uint32_t* roff = w.off.as<uint32_t>();
+  if (vmap) {
+    roff_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+    roff = roff_.as<uint32_t>();
+  }
+  exclusive_scan_u32(dout_.as<uint32_t>(), roff, nc + 1);
+  const uint32_t mc2 = read_u32(roff + nc);
   const uint32_t max_dout = reduce_max_u32(dout_.as<uint32_t>(), nc);
   const size_t mm = std::max<uint32_t>(mc2, 1);
   w.adj.ensure(sizeof(uint32_t) * mm);
   w.src.ensure(sizeof(uint32_t) * mm);
   w.orig.ensure(sizeof(uint32_t) * mm);
   w.sup.ensure(sizeof(int32_t) * mm);
-  row_compact(cv->off_d, cv->adj_d, cv->orig, cv->sup, cv->state, nc, w.off.as<uint32_t>(),
-              w.adj.as<uint32_t>(), w.src.as<uint32_t>(), w.orig.as<uint32_t>(),
-              carry_sup ? w.sup.as<int32_t>() : nullptr, stream_);
+  row_compact(cv->off_d, cv->adj_d, cv->orig, cv->sup, cv->state, nc, roff, w.adj.as<uint32_t>(),
+              w.src.as<uint32_t>(), w.orig.as<uint32_t>(), carry_sup ? w.sup.as<int32_t>() : nullptr, stream_);
+  if (vmap) {  // rank order is kept, so rows stay sorted and u < v stays u < v
+    relabel_edges(w.adj.as<uint32_t>(), w.src.as<uint32_t>(), mc2, vmap, stream_);
+    relabel_offsets(roff, vmap, nc, w.off.as<uint32_t>(), stream_);
+    cv->nc = nc2;
+  }
   cv->off_d = w.off.as<uint32_t>();
   cv->adj_d = w.adj.as<uint32_t>();
   cv->src = w.src.as<uint32_t>();
@@ -335,104 +365,149 @@ void Solver::compact_core(CoreView* cv, bool carry_sup) {
   cv->work = t;
   build_undirected_view(cv);
   fill_u32(cv->state, mc2, kNone, stream_);
+  if (vmap && cv->dense) make_dense(cv);
   if (!carry_sup) count_support(cv, nullptr);
+}
+
+bool Solver::try_redensify(CoreView* cv) {
+  const uint32_t nc = cv->nc;
+  vflag_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  vmap_.ensure(sizeof(uint32_t) * (size_t(nc) + 1));
+  fill_u32(vflag_.as<uint32_t>(), uint64_t(nc) + 1, 0, stream_);
+  live_vertex_flags(cv->state, cv->src, cv->adj_d, cv->mc, vflag_.as<uint32_t>(), stream_);
+  exclusive_scan_u32(vflag_.as<uint32_t>(), vmap_.as<uint32_t>(), nc + 1);
+  const uint32_t nc2 = read_u32(vmap_.as<uint32_t>() + nc);
+  if (uint64_t(nc2) * 4 > uint64_t(nc) * 3) return false;  // rows would shrink by less than a quarter
+  compact_core(cv, false, vmap_.as<uint32_t>(), nc2);
+  return true;
 }
 
 // ------------------------------------------------------------- (d) peel ---
 
-// Whether a frontier of c edges (supports < thr) is cheaper to drop with an
-// exact recount of the alive edges than to walk triangle by triangle. Dense
-// cores: a recount is alive * ws word ANDs, a frontier walk c * (ws + ~8 thr)
-// random lookups and atomics, so a few percent of the alive edges already
-// favour the recount. Sorted-row cores pay a compaction and a rebuild first:
-// the reference's third (src/kc.cu:988) scaled by recount_ratio_.
-bool Solver::drop_frontier(const CoreView& cv, uint64_t c, uint64_t alive, int32_t thr) const {
-  if (cv.dense) return double(c) * (cv.ws + dense_tri_cost_ * double(thr)) >= double(alive) * cv.ws;
-  return double(c) * sparse_drop_ratio_ >= double(alive);
-}
-
+// The round rule decides on the device whether a frontier of c edges
+// (supports < thr) is cheaper to drop with an exact recount of the alive
+// edges than to walk triangle by triangle. Dense cores: a recount is
+// alive * ws word ANDs, a frontier walk c * (ws + tri_cost * thr) random
+// lookups and atomics, so a few percent of the alive edges already favour
+// the recount. Sorted-row cores pay a compaction and a rebuild first: the
+// reference's third (src/kc.cu:988) scaled by drop_ratio.
 // Truss levels on the core view, upward from threshold thr0 (an edge stays
 // while its support >= thr, i.e. in the (thr+2)-truss). Empty levels are
 // skipped by jumping to the minimum live support. With early_ans != 0, a
 // first level of exactly early_ans edges at kmax early_k ends the search
 // (src/kc.cu:1240-1241).
+RoundRule Solver::round_rule(const CoreView& cv, int32_t thr) const {
+  RoundRule rule{};
+  rule.dense = cv.dense ? 1u : 0u;
+  rule.ws = cv.ws;
+  rule.mc = cv.mc;
+  rule.thr = thr;
+  rule.tri_cost = dense_tri_cost_;
+  rule.drop_ratio = sparse_drop_ratio_;
+  return rule;
+}
+
+void Solver::start_rounds(CoreView* cv, int32_t thr, uint32_t r, uint64_t alive, bool reset) {
+  uint32_t* q0 = eq0_.as<uint32_t>();
+  zero_u32(&ctl_->qcount[0]);
+  edge_frontier_init(cv->state, cv->sup, cv->mc, thr, q0, &ctl_->qcount[0], stream_);
+  round_start(ctl_, r, alive, round_rule(*cv, thr), reset, stream_);
+  round_stamp(q0, eq1_.as<uint32_t>(), ctl_, uint32_t(thr), cv->state, cv->orig, cv->del_thr, cv->src, cv->adj_d,
+              cv->dense ? cv->fbits : nullptr, cv->ws, stream_);
+}
+
+void Solver::enqueue_rounds(CoreView* cv, int32_t thr, int count) {
+  uint32_t* q0 = eq0_.as<uint32_t>();
+  uint32_t* q1 = eq1_.as<uint32_t>();
+  const RoundRule rule = round_rule(*cv, thr);
+  const double avg = cv->nc ? double(2ull * cv->mc) / cv->nc : 0.0;
+  const int lanes = lanes_peel_ ? lanes_peel_ : auto_lanes(avg);
+  for (int i = 0; i < count; ++i) {
+    if (cv->dense)
+      dense_peel_round(q0, q1, ctl_, thr, cv->src, cv->adj_d, cv->bits, cv->fbits, cv->ws, cv->eidm, cv->nc,
+                       cv->bits0, cv->pref0, cv->off_u, cv->eid_u, cv->sup, stream_);
+    else
+      sparse_peel_round(q0, q1, ctl_, thr, lanes, cv->off_u, cv->adj_u, cv->eid_u, cv->src, cv->adj_d, cv->state,
+                        cv->sup, stream_);
+    round_advance(q0, q1, ctl_, cv->src, cv->adj_d, cv->dense ? cv->bits : nullptr,
+                  cv->dense ? cv->fbits : nullptr, cv->ws, rule, stream_);
+    round_stamp(q0, q1, ctl_, uint32_t(thr), cv->state, cv->orig, cv->del_thr, cv->src, cv->adj_d,
+                cv->dense ? cv->fbits : nullptr, cv->ws, stream_);
+  }
+}
+
 TrussOutcome Solver::truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k,
                                   bool exact_early) {
+  constexpr int kMaxBatch = 32;
   TrussOutcome out;
   if (cv->mc == 0) return out;
-  uint32_t* qc = ctl_->qcount;
-  uint32_t* q[2] = {eq0_.as<uint32_t>(), eq1_.as<uint32_t>()};
   uint64_t alive = cv->mc;
   uint32_t r = 1;
   int32_t thr = thr0;
-  bool first = true;
-  int cur = 0;
-  auto frontier = [&]() -> uint32_t {
-    cur = 0;
-    zero_u32(&qc[0]);
-    edge_frontier_init(cv->state, cv->sup, cv->mc, thr, q[0], &qc[0], stream_);
-    return read_u32(&qc[0]);
-  };
+  bool first = true, reset = true;
   for (;;) {
     if (!cv->dense && 2 * alive <= cv->mc) {  // half the edges are dead: compact, keep supports
       compact_core(cv, true);
       ++out.compactions;
       r = 1;
     }
-    uint32_t c = frontier();
-    while (c) {
+    start_rounds(cv, thr, r, alive, reset);
+    reset = false;
+    // Rounds run on the device in batches; the host only looks at the
+    // control block between batches and acts when a round stops walking.
+    int batch = 2;
+    uint64_t queued = 0;  // every walked round removes an edge
+    for (;;) {
+      if (queued > uint64_t(cv->mc) + 2 * kMaxBatch) fail("edge peel did not converge");
+      enqueue_rounds(cv, thr, batch);
+      queued += batch;
+      const Ctl h = read_ctl();
       if (trace_)
-        std::fprintf(stderr, "[kt] edge-peel thr=%d alive=%llu mc=%u frontier=%u dense=%d drop=%d\n", thr,
-                     (unsigned long long)alive, cv->mc, c, int(cv->dense), int(drop_frontier(*cv, c, alive, thr)));
-      edge_stamp_thr(q[cur], &qc[cur], c, r, uint32_t(thr), cv->state, cv->orig, cv->del_thr, stream_);
-      ++out.rounds;
-      if (c == alive) {  // every live edge goes: the level is empty
-        ++r;
+        std::fprintf(stderr, "[kt] edge-peel thr=%d batch=%d action=%u alive=%llu mc=%u frontier=%u round=%u\n", thr,
+                     batch, h.action, h.alive, cv->mc, h.qcount[h.cur], h.round);
+      if (h.action == kRoundPeel) {
+        batch = std::min(batch * 2, kMaxBatch);
+        continue;
+      }
+      out.rounds = h.rounds_done;
+      alive = h.alive;
+      r = h.round + 1;
+      if (h.action == kRoundIdle) break;
+      if (h.action == kRoundEmpty) {  // every live edge went: the level is empty
         alive = 0;
         break;
       }
-      if (drop_frontier(*cv, c, alive, thr)) {  // drop the frontier, recount the rest exactly
+      if (h.action == kRoundDrop) {  // drop the frontier, recount the rest exactly
         if (cv->dense) {
-          dense_clear(q[cur], &qc[cur], c, cv->src, cv->adj_d, cv->bits, cv->ws, stream_);
-          dense_support(cv->src, cv->adj_d, cv->mc, cv->bits, cv->ws, cv->state, cv->sup, stream_);
-          alive -= c;
-          ++r;
+          if (try_redensify(cv)) {  // smaller matrix on the vertices left, exact recount
+            if (cv->mc != alive) fail("edge peel bookkeeping mismatch (redensify)");
+            r = 1;
+          } else {
+            frontier_clear(eq0_.as<uint32_t>(), eq1_.as<uint32_t>(), ctl_, cv->src, cv->adj_d, cv->bits,
+                           cv->fbits, cv->ws, stream_);
+            dense_support(cv->src, cv->adj_d, cv->mc, cv->bits, cv->ws, cv->state, cv->sup, stream_);
+          }
         } else {
           compact_core(cv, false);
           alive = cv->mc;
           r = 1;
         }
-        ++out.compactions;
-        c = frontier();
-        continue;
-      }
-      zero_u32(&qc[cur ^ 1]);
-      if (cv->dense) {
-        dense_peel(q[cur], &qc[cur], c, r, thr, cv->src, cv->adj_d, cv->bits0, cv->pref0, cv->bits, cv->ws,
-                   cv->off_u, cv->eid_u, cv->state, cv->sup, q[cur ^ 1], &qc[cur ^ 1], stream_);
-        dense_clear(q[cur], &qc[cur], c, cv->src, cv->adj_d, cv->bits, cv->ws, stream_);
-      } else {
-        const double avg = cv->nc ? double(2ull * cv->mc) / cv->nc : 0.0;
-        const int lanes = lanes_peel_ ? lanes_peel_ : auto_lanes(avg);
-        peel_process(cv->off_u, cv->adj_u, cv->eid_u, cv->src, cv->adj_d, q[cur], &qc[cur], c, r, thr, lanes,
-                     cv->state, cv->sup, q[cur ^ 1], &qc[cur ^ 1], stream_);
-      }
-      alive -= c;
-      ++r;
-      cur ^= 1;
-      c = read_u32(&qc[cur]);
-      if (!cv->dense && c && 2 * alive <= cv->mc) {
+      } else {  // half the edges are dead: compact before stamping the next frontier
         compact_core(cv, true);
-        ++out.compactions;
         if (cv->mc != alive) fail("edge peel bookkeeping mismatch");
         r = 1;
-        c = frontier();
       }
+      ++out.compactions;
+      start_rounds(cv, thr, r, alive, false);
+      batch = 2;
+      queued = 0;
     }
     if (alive == 0) break;
+    if (alive > cv->mc) fail("edge peel bookkeeping mismatch (live count)");
     fill_u32(&ctl_->minsup, 1, kNone, stream_);
     edge_min_support(cv->state, cv->sup, cv->mc, &ctl_->minsup, stream_);
     const uint32_t ms = read_u32(&ctl_->minsup);
+    if (ms == kNone || ms >= 0x7FFFFFFFu) fail("edge peel bookkeeping mismatch (no live edge)");
     out.kmax = ms + 2;
     out.count = alive;
     out.floor_thr = thr;
@@ -534,7 +609,7 @@ void Solver::last_stats(kt_stats* out) {
   if (!have_truss_) fail("no solve has completed on this context");
   std::memset(out, 0, sizeof(*out));
   const CoreView& cv = last_core_;
-  out->n_core = cv.nc;
+  out->n_core = cv.nc0;
   out->m_core = cv.mc0;
   out->truss_edges = last_truss_;
   if (cv.mc0 == 0) return;
