@@ -16,6 +16,13 @@ inline int grid_for(uint64_t items, int per_block = kBlock) {
 
 __device__ __forceinline__ int lane64() { return int(__lane_id()); }
 
+// Orders LDS writes and reads between the lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Wave-aggregated queue append. Must be reached by every lane of the wave.
 template <typename T>
 __device__ __forceinline__ void wave_append(bool pred, T item, T* q, uint32_t* count) {

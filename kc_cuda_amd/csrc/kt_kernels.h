@@ -36,16 +36,14 @@ void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, uint32_t B, uin
 
 // (b) vertex peeling (k-core). lvl[v] == kNone: alive; otherwise the
 //     threshold at which v was removed. deg holds alive-neighbour counts.
-void core_frontier_init(const uint32_t* lvl, const uint32_t* deg, uint32_t n, uint32_t k,
-                        uint32_t* q, uint32_t* qcount, hipStream_t s);
-// Device-driven rounds: core_round_start reads the frontier size from
-// ctl->qcount[0] (filled by core_frontier_init into q0); each core_round
-// stamps the frontier, decrements the alive neighbours' degrees, queues the
-// ones that fall below k and switches queues. ctl->removed counts the
-// removed vertices; ctl->action turns idle once a frontier is empty.
+// Device-driven rounds: each core_round queues and stamps (lvl = k) every
+// alive vertex of degree < k, then takes one degree off every neighbour of
+// the queued vertices. ctl->removed counts the removed vertices;
+// ctl->action turns idle after a round that found nothing to remove.
+// q holds n entries.
 void core_round_start(Ctl* ctl, hipStream_t s);
-void core_round(const uint64_t* off, const uint32_t* adj, uint32_t* q0, uint32_t* q1, Ctl* ctl,
-                uint32_t k, uint32_t* lvl, uint32_t* deg, HeavyList hl, hipStream_t s);
+void core_round(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t* q, Ctl* ctl, uint32_t k,
+                uint32_t* lvl, uint32_t* deg, HeavyList hl, hipStream_t s);
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl,
                         uint32_t n, uint32_t* deg, HeavyList hl, hipStream_t s);
 // lvl[v] = (lvl[v] > k) ? kNone : (lvl[v] == kNone ? kNone : k) -- used to

@@ -231,43 +231,41 @@ __global__ void k_deg_from_off(const uint64_t* off, uint32_t n, uint32_t* deg) {
 
 // ----------------------------------------------------- (b) vertex peel ---
 
-__global__ __launch_bounds__(kBlock) void k_core_frontier(const uint32_t* __restrict__ lvl,
-                                                          const uint32_t* __restrict__ deg, uint32_t n,
-                                                          uint32_t k, uint32_t* q, uint32_t* qcount) {
-  tile_compact(
-      n, [&](uint64_t v) { return lvl[v] == kNone && deg[v] < k; },
-      [&](uint32_t at, uint64_t v) { q[at] = uint32_t(v); }, qcount);
-}
-
-// Device-driven vertex-peel rounds: the frontier is q[ctl->cur] with
-// ctl->qcount[cur] entries; every kernel of a round does nothing once a
-// round has found an empty frontier (action idle).
+// Device-driven vertex-peel rounds (reference: src/kc.cu:653-884, where each
+// round scans a mask and decrements with returned atomics). A round here is
+//   sweep  queue and stamp every alive vertex of degree < k (lvl = k);
+//   push   every alive neighbour of a queued vertex loses one degree (atomics
+//          with no return value: who falls below k is the next sweep's job);
+//   close  count the removed vertices; an empty sweep ends the peel.
+// The frontier lives in q with ctl->qcount[0] entries; every kernel of a
+// round does nothing once the peel has ended (action idle).
 __global__ void k_vround_start(Ctl* ctl) {
   if (threadIdx.x != 0) return;
-  ctl->cur = 0;
-  ctl->qcount[1] = 0;
+  ctl->qcount[0] = 0;
   ctl->pad[1] = 0;  // heavy-row count
   ctl->removed = 0;
-  ctl->action = ctl->qcount[0] ? kRoundPeel : kRoundIdle;
+  ctl->action = kRoundPeel;
 }
 
-__global__ void k_core_stamp(const uint32_t* q0, const uint32_t* q1, const Ctl* ctl, uint32_t k, uint32_t* lvl) {
+__global__ __launch_bounds__(kBlock) void k_core_sweep(uint32_t* lvl, const uint32_t* __restrict__ deg, uint32_t n,
+                                                       uint32_t k, uint32_t* q, Ctl* ctl) {
   if (ctl->action != kRoundPeel) return;
-  const uint32_t cur = ctl->cur, cnt = ctl->qcount[cur];
-  const uint32_t* q = cur ? q1 : q0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
-    lvl[q[i]] = k;
+  tile_compact(
+      n, [&](uint64_t v) { return lvl[v] == kNone && deg[v] < k; },
+      [&](uint32_t at, uint64_t v) {
+        q[at] = uint32_t(v);
+        lvl[v] = k;
+      },
+      &ctl->qcount[0]);
 }
 
-// Close a round: count what it removed, make the next queue current.
-__global__ void k_vround_advance(Ctl* ctl) {
+__global__ void k_vround_close(Ctl* ctl) {
   if (threadIdx.x != 0 || ctl->action != kRoundPeel) return;
-  const uint32_t cur = ctl->cur;
-  ctl->removed += ctl->qcount[cur];
-  ctl->qcount[cur] = 0;
-  ctl->cur = cur ^ 1u;
+  const uint32_t c = ctl->qcount[0];
+  ctl->removed += c;
+  ctl->qcount[0] = 0;
   ctl->pad[1] = 0;
-  ctl->action = ctl->qcount[cur ^ 1u] ? kRoundPeel : kRoundIdle;
+  if (c == 0) ctl->action = kRoundIdle;
 }
 
 // Rows longer than kHeavyRow slots are not walked by one wave: the light
@@ -321,66 +319,87 @@ __device__ __forceinline__ uint32_t slice_row(const uint32_t* slices, uint32_t h
   return lo;
 }
 
-__device__ __forceinline__ void core_visit(uint32_t w, bool valid, uint32_t k, const uint32_t* __restrict__ lvl,
-                                           uint32_t* deg, uint32_t* qnext, uint32_t* qnext_count) {
-  bool crossed = false;
-  if (valid && lvl[w] == kNone) crossed = (atomicSub(&deg[w], 1u) == k);
-  wave_append<uint32_t>(crossed, w, qnext, qnext_count);
-}
-
-__global__ __launch_bounds__(kBlock) void k_core_process_heavy(
-    const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj, const uint32_t* heavy,
-    const uint32_t* heavy_count, const uint32_t* __restrict__ slices, uint32_t k,
-    const uint32_t* __restrict__ lvl, uint32_t* deg, uint32_t* q0, uint32_t* q1, Ctl* ctl) {
+__global__ __launch_bounds__(kBlock) void k_core_push_heavy(const uint64_t* __restrict__ off,
+                                                            const uint32_t* __restrict__ adj, const uint32_t* heavy,
+                                                            const uint32_t* heavy_count,
+                                                            const uint32_t* __restrict__ slices,
+                                                            const uint32_t* __restrict__ lvl, uint32_t* deg,
+                                                            const Ctl* ctl) {
   if (ctl->action != kRoundPeel) return;
   const uint32_t hc = *heavy_count;
   if (hc == 0) return;
-  const uint32_t cur = ctl->cur;
-  uint32_t* qnext = cur ? q0 : q1;
-  uint32_t* qnext_count = &ctl->qcount[cur ^ 1u];
   const uint32_t total = slices[hc];
-  const int lane = lane64();
   for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
     const uint32_t h = slice_row(slices, hc, t);
     const uint32_t u = heavy[h];
     const uint64_t b = off[u] + uint64_t(t - slices[h]) * kSlice;
     const uint64_t e = min(off[u + 1], b + kSlice);
-    for (uint64_t j0 = b + (threadIdx.x & ~63u); j0 < e; j0 += blockDim.x) {
-      const uint64_t j = j0 + lane;
-      core_visit(j < e ? adj[j] : 0, j < e, k, lvl, deg, qnext, qnext_count);
+    for (uint64_t j = b + threadIdx.x; j < e; j += blockDim.x) {
+      const uint32_t w = adj[j];
+      if (lvl[w] == kNone) atomicSub(&deg[w], 1u);
     }
   }
 }
 
-// One wave per removed vertex: every still-alive neighbour loses one degree;
-// the decrement that takes it from k to k-1 queues it (exactly once).
-__global__ __launch_bounds__(kBlock) void k_core_process(
-    const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj, uint32_t* q0, uint32_t* q1, Ctl* ctl,
-    uint32_t k, const uint32_t* __restrict__ lvl, uint32_t* deg, uint32_t* heavy, uint32_t* heavy_count) {
+// Queued vertices are taken in groups of 64: a wave scans the group's row
+// lengths and deals the concatenated slots out evenly, so short rows leave no
+// lane idle. Each group is shared by P waves (part p takes the 64-slot chunks
+// p, p + P, ...), with P sized from the queue so that small frontiers of long
+// rows still occupy the whole grid. Rows longer than kHeavyRow go to the
+// heavy list for the sliced kernel.
+__global__ __launch_bounds__(kBlock) void k_core_push(const uint64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ adj,
+                                                      const uint32_t* __restrict__ q, const Ctl* ctl,
+                                                      const uint32_t* __restrict__ lvl, uint32_t* deg,
+                                                      uint32_t* heavy, uint32_t* heavy_count) {
   if (ctl->action != kRoundPeel) return;
-  const uint32_t cur = ctl->cur, cnt = ctl->qcount[cur];
-  const uint32_t* q = cur ? q1 : q0;
-  uint32_t* qnext = cur ? q0 : q1;
-  uint32_t* qnext_count = &ctl->qcount[cur ^ 1u];
-  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const uint32_t cnt = ctl->qcount[0];
+  __shared__ uint64_t s_b[kBlock];
+  __shared__ uint32_t s_inc[kBlock];
   const int lane = lane64();
-  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < cnt; i += waves) {
-    const uint32_t u = q[i];
-    const uint64_t b = off[u], e = off[u + 1];
-    if (e - b > kHeavyRow) {
-      if (lane == 0) push_heavy(u, heavy, heavy_count);
-      continue;
+  const uint32_t wbase = threadIdx.x & ~uint32_t(kWave - 1);
+  const uint32_t nw = gridDim.x * (kBlock / kWave);
+  const uint32_t groups = (cnt + kWave - 1) / kWave;
+  const uint32_t parts = groups ? max(1u, min(uint32_t(kWave), (nw + groups - 1) / groups)) : 1u;
+  const uint64_t items = uint64_t(groups) * parts;
+  for (uint64_t it = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; it < items; it += nw) {
+    const uint32_t grp = uint32_t(it / parts), part = uint32_t(it - uint64_t(grp) * parts);
+    const uint32_t i = grp * kWave + uint32_t(lane);
+    uint32_t u = 0, len = 0;
+    uint64_t b = 0;
+    bool hv = false;
+    if (i < cnt) {
+      u = q[i];
+      b = off[u];
+      const uint64_t l = off[u + 1] - b;
+      hv = l > kHeavyRow;
+      len = hv ? 0u : uint32_t(l);
     }
-    for (uint64_t j0 = b; j0 < e; j0 += kWave) {
-      const uint64_t j = j0 + lane;
-      bool crossed = false;
-      uint32_t w = 0;
-      if (j < e) {
-        w = adj[j];
-        if (lvl[w] == kNone) crossed = (atomicSub(&deg[w], 1u) == k);
+    wave_append<uint32_t>(hv && part == 0, u, heavy, heavy_count);
+    uint32_t inc = len;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o);
+      if (lane >= o) inc += y;
+    }
+    const uint32_t total = __shfl(inc, kWave - 1);
+    s_b[threadIdx.x] = b;
+    s_inc[threadIdx.x] = inc;
+    wave_sync();
+    for (uint32_t j = part * kWave + uint32_t(lane); j < total; j += kWave * parts) {
+      uint32_t lo = 0, hi = kWave - 1;  // owner: first lane whose inclusive length exceeds j
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_inc[wbase + mid] > j)
+          hi = mid;
+        else
+          lo = mid + 1;
       }
-      wave_append<uint32_t>(crossed, w, qnext, qnext_count);
+      const uint32_t before = lo ? s_inc[wbase + lo - 1] : 0u;
+      const uint32_t w = adj[s_b[wbase + lo] + (j - before)];
+      if (lvl[w] == kNone) atomicSub(&deg[w], 1u);
     }
+    wave_sync();
   }
 }
 
@@ -1124,20 +1143,16 @@ void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStr
   if (n) k_deg_from_off<<<grid_for(n), kBlock, 0, s>>>(off, n, deg);
 }
 
-void core_frontier_init(const uint32_t* lvl, const uint32_t* deg, uint32_t n, uint32_t k, uint32_t* q,
-                        uint32_t* qcount, hipStream_t s) {
-  if (n) k_core_frontier<<<grid_tiles(n), kBlock, 0, s>>>(lvl, deg, n, k, q, qcount);
-}
 void core_round_start(Ctl* ctl, hipStream_t s) { k_vround_start<<<1, kWave, 0, s>>>(ctl); }
-void core_round(const uint64_t* off, const uint32_t* adj, uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t k,
+void core_round(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t* q, Ctl* ctl, uint32_t k,
                 uint32_t* lvl, uint32_t* deg, HeavyList hl, hipStream_t s) {
-  constexpr int kStampGrid = 512, kProcessGrid = 2048;
-  k_core_stamp<<<kStampGrid, kBlock, 0, s>>>(q0, q1, ctl, k, lvl);
-  k_core_process<<<kProcessGrid, kBlock, 0, s>>>(off, adj, q0, q1, ctl, k, lvl, deg, hl.items, hl.count);
+  constexpr int kPushGrid = 2048;
+  if (!n) return;
+  k_core_sweep<<<grid_tiles(n), kBlock, 0, s>>>(lvl, deg, n, k, q, ctl);
+  k_core_push<<<kPushGrid, kBlock, 0, s>>>(off, adj, q, ctl, lvl, deg, hl.items, hl.count);
   k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices, ctl);
-  k_core_process_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, hl.items, hl.count, hl.slices, k, lvl, deg, q0, q1,
-                                                     ctl);
-  k_vround_advance<<<1, kWave, 0, s>>>(ctl);
+  k_core_push_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, hl.items, hl.count, hl.slices, lvl, deg, ctl);
+  k_vround_close<<<1, kWave, 0, s>>>(ctl);
 }
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, uint32_t n,
                         uint32_t* deg, HeavyList hl, hipStream_t s) {

@@ -35,12 +35,6 @@ __device__ __forceinline__ uint32_t decide_round(uint32_t c, unsigned long long 
   return drop ? kRoundDrop : kRoundPeel;
 }
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __global__ void k_round_start(Ctl* ctl, uint32_t r, unsigned long long alive, RoundRule rule, uint32_t reset) {
   if (threadIdx.x != 0) return;
   const uint32_t c = ctl->qcount[0];

@@ -43,23 +43,19 @@ constexpr uint32_t kMinCut = 256;  // smaller alive sets stay on their level
 // between batches only.
 uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
   constexpr int kMaxBatch = 32;
+  if (g.n == 0) return 0;
   q0_.ensure(sizeof(uint32_t) * (size_t(g.n) + 1));
-  q1_.ensure(sizeof(uint32_t) * (size_t(g.n) + 1));
-  uint32_t* q0 = q0_.as<uint32_t>();
-  uint32_t* q1 = q1_.as<uint32_t>();
-  zero_u32(&ctl_->qcount[0]);
-  core_frontier_init(g.lvl, g.deg, g.n, k, q0, &ctl_->qcount[0], stream_);
+  uint32_t* q = q0_.as<uint32_t>();
   core_round_start(ctl_, stream_);
   const HeavyList hl = heavy();
   int batch = 2;
-  uint64_t queued = 0;  // every round with a frontier removes a vertex
+  uint64_t queued = 0;  // every round but the last removes a vertex
   for (;;) {
-    for (int i = 0; i < batch; ++i) core_round(g.off, g.adj, q0, q1, ctl_, k, g.lvl, g.deg, hl, stream_);
+    for (int i = 0; i < batch; ++i) core_round(g.off, g.adj, g.n, q, ctl_, k, g.lvl, g.deg, hl, stream_);
     queued += batch;
     const Ctl h = read_ctl();
     if (trace_)
-      std::fprintf(stderr, "[kt] vertex-peel level_n=%u k=%u batch=%d removed=%llu frontier=%u\n", g.n, k, batch,
-                   h.removed, h.action == kRoundPeel ? h.qcount[h.cur] : 0u);
+      std::fprintf(stderr, "[kt] vertex-peel level_n=%u k=%u batch=%d removed=%llu\n", g.n, k, batch, h.removed);
     if (h.action != kRoundPeel) {
       if (h.removed > g.n) fail("vertex peel bookkeeping mismatch");
       return h.removed;
