@@ -106,12 +106,16 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
  * KT_FORCE_EDGE_COUNT selects the per-edge support kernel instead of the LDS
  * bitmap one, KT_FORCE_ATOMIC_BUILD the atomic CSR build instead of the
  * bucketed one, KT_FORCE_SPARSE / KT_FORCE_DENSE the sorted-row or the
- * bit-matrix core representation (default: by density). Every variant
- * computes the same result. */
+ * bit-matrix core representation (default: by density), KT_FORCE_LEVEL_WALK
+ * the level-by-level first pass on bit-matrix cores (default: 16 levels are
+ * walked, then thresholds are probed), KT_FORCE_PROBES threshold probes right
+ * after the first level. Every variant computes the same result. */
 #define KT_FORCE_EDGE_COUNT 1
 #define KT_FORCE_ATOMIC_BUILD 2
 #define KT_FORCE_SPARSE 4
 #define KT_FORCE_DENSE 8
+#define KT_FORCE_LEVEL_WALK 16
+#define KT_FORCE_PROBES 32
 int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
 
 /* Byte-model terms of the last solve (one extra device pass, untimed). */

@@ -55,22 +55,66 @@ struct Ctl {
   uint32_t stamped;      // the current frontier has been stamped
   unsigned long long alive;    // live, unstamped edges / alive vertices
   unsigned long long removed;  // vertices removed by the vertex peel
+  // Truss levels, also advanced on the device: the threshold of the level
+  // being peeled (an edge stays while its support >= thr) and the record of
+  // the last completed level.
+  int32_t thr;
+  uint32_t levels;
+  uint32_t first;        // no level completed yet
+  uint32_t level_kmax;   // min support of the last completed level + 2
+  unsigned long long level_count;
+  int32_t level_floor;   // threshold of the last completed level
+  uint32_t early_k;      // early exit: a first level of early_ans edges ...
+  unsigned long long early_ans;
+  uint32_t exact_early;  // ... at kmax early_k (or any kmax when set)
+  uint32_t max_levels;   // stop after this many completed levels (0: no limit)
 };
 
-constexpr uint32_t kRoundIdle = 0;     // empty frontier: the level is done
+// ctl->err bit: a level ended with live edges but no live support (or the
+// reverse); the edge-peel bookkeeping is inconsistent
+constexpr uint32_t kErrPeelCount = 1u << 8;
+
+constexpr uint32_t kRoundIdle = 0;     // vertex peel: nothing left to remove
 constexpr uint32_t kRoundPeel = 1;     // walk the frontier's triangles
 constexpr uint32_t kRoundDrop = 2;     // drop the frontier, recount the rest
 constexpr uint32_t kRoundEmpty = 3;    // the frontier is every live edge
 constexpr uint32_t kRoundCompact = 4;  // compact the edge set first (unstamped)
+constexpr uint32_t kRoundMinSup = 5;   // level done: reduce the live supports
+constexpr uint32_t kRoundSweep = 6;    // next level: collect its frontier
+constexpr uint32_t kRoundDone = 7;     // search finished (early exit, no edges)
 
 // Host-set parameters of the device round decision (drop or walk a frontier).
 struct RoundRule {
   uint32_t dense;     // bit-matrix core
   uint32_t ws;        // words per bit-matrix row
   uint32_t mc;        // edges of the current edge set (compaction rule)
-  int32_t thr;        // level threshold
+  uint32_t pad;
   double tri_cost;    // dense: cost of one triangle visit in word ANDs
   double drop_ratio;  // sparse: drop when frontier * ratio >= alive
+};
+
+// How a truss search ends: early exit on a first level of early_ans edges at
+// kmax early_k (any kmax with exact_early), or after max_levels completed
+// levels (0: no limit).
+struct SearchRule {
+  unsigned long long early_ans;
+  uint32_t early_k;
+  uint32_t exact_early;
+  uint32_t max_levels;
+  uint32_t pad;
+};
+
+// Arrays of a core view that the edge-peel round kernels touch.
+struct PeelArgs {
+  uint32_t *q0, *q1;
+  const uint32_t *src, *dst, *orig;  // oriented edges and their base ids
+  uint32_t *state, *del_thr;
+  int32_t* sup;
+  uint32_t mc, nc, ws;
+  const uint32_t *off_u, *adj_u, *eid_u;  // undirected view (sorted rows)
+  uint64_t *bits, *fbits;                 // dense views: live / frontier bits
+  const uint64_t* bits0;
+  const uint32_t *pref0, *eidm;
 };
 
 }  // namespace kt

@@ -110,23 +110,18 @@ void dense_rows(const uint32_t* off_u, const uint32_t* adj_u, uint32_t nc, uint3
 void dense_support(const uint32_t* src, const uint32_t* dst, uint32_t mc, const uint64_t* bits,
                    uint32_t ws, const uint32_t* state, int32_t* sup, hipStream_t s);
 
-// Device-driven edge-peel rounds (kt_peel.hip). q0/q1 are the two frontier
-// queues; ctl->cur says which one holds the current frontier.
-void round_start(Ctl* ctl, uint32_t r, uint64_t alive, const RoundRule& rule, bool reset, hipStream_t s);
-void round_stamp(uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t thr, uint32_t* state, const uint32_t* orig,
-                 uint32_t* del_thr, const uint32_t* src, const uint32_t* dst, uint64_t* fbits, uint32_t ws,
-                 hipStream_t s);
-void round_advance(uint32_t* q0, uint32_t* q1, Ctl* ctl, const uint32_t* src, const uint32_t* dst,
-                   uint64_t* bits, uint64_t* fbits, uint32_t ws, const RoundRule& rule, hipStream_t s);
-void frontier_clear(const uint32_t* q0, const uint32_t* q1, const Ctl* ctl, const uint32_t* src,
-                    const uint32_t* dst, uint64_t* bits, uint64_t* fbits, uint32_t ws, hipStream_t s);
-void dense_peel_round(uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, const uint32_t* src,
-                      const uint32_t* dst, const uint64_t* bits, const uint64_t* fbits, uint32_t ws,
-                      const uint32_t* eidm, uint32_t nc, const uint64_t* bits0, const uint32_t* pref0,
-                      const uint32_t* off_u, const uint32_t* eid_u, int32_t* sup, hipStream_t s);
-void sparse_peel_round(uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, int lanes, const uint32_t* off_u,
-                       const uint32_t* adj_u, const uint32_t* eid_u, const uint32_t* src,
-                       const uint32_t* dst, const uint32_t* state, int32_t* sup, hipStream_t s);
+// Device-driven edge-peel rounds and truss levels (kt_peel.hip). q0/q1 are
+// the two frontier queues; ctl->cur says which one holds the current
+// frontier, ctl->thr the level threshold, ctl->action what comes next.
+// Starts a frontier already collected into q0 (count in ctl->qcount[0]) at
+// threshold thr with `alive` live edges; reset also starts a new search
+// (level record cleared, search rule set). Stamps nothing.
+void round_start(Ctl* ctl, uint32_t r, uint64_t alive, int32_t thr, const RoundRule& rule, bool reset,
+                 const SearchRule& search, hipStream_t s);
+void round_stamp(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, hipStream_t s);
+// `count` rounds of (peel, advance, stamp)
+void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, int count, hipStream_t s);
+void frontier_clear(const PeelArgs& a, const Ctl* ctl, hipStream_t s);
 void dense_eidm(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, uint32_t nc,
                 uint32_t* eidm, hipStream_t s);
 // Vertex compaction of a core view: flag the endpoints of live edges (flag

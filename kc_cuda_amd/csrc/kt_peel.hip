@@ -1,16 +1,21 @@
 // Device-driven edge-peel rounds (truss levels) on CDNA4.
 //
 // A level removes, round by round, every edge whose support is below the
-// threshold (reference: src/kc.cu:886-1056, __delete_edge / ktruss). Here a
-// round is three launches that read their frontier, its size and the round
-// stamp from the device control block:
+// threshold (reference: src/kc.cu:886-1056, __delete_edge / ktruss), and the
+// search moves from level to level (src/kc.cu:1140-1309). Here both run on
+// the device: a round is three launches that read what to do, the frontier,
+// its size, the round stamp and the threshold from the control block.
 //   peel     walk the triangles of the frontier, decrement the partners,
 //            append edges that cross below the threshold to the next queue;
+//            at a level change: collect the new level's frontier;
 //   advance  clear the frontier from the live bit matrix (dense cores) and, in
 //            the last workgroup, switch queues and decide the next round;
-//   stamp    stamp the next frontier (state, deletion threshold, frontier bits).
-// The host enqueues rounds in batches and reads the control block once per
-// batch; a round whose action is not "peel" does nothing.
+//   stamp    stamp the current frontier (state, deletion threshold, frontier
+//            bits); at a level's end: reduce the live supports and, in the
+//            last workgroup, record the level and set the next threshold.
+// The host enqueues rounds in batches, reads the control block once per batch
+// and steps in only to drop a frontier, compact, or collect the answer; a
+// kernel whose action does not concern it does nothing.
 #include "kt_device.h"
 #include "kt_kernels.h"
 
@@ -19,46 +24,112 @@ namespace {
 
 // Grids of the round kernels. Stamp and advance end with one ticket atomic
 // per workgroup, so they stay small; their loops stride over the frontier.
-constexpr int kStampGrid = 128;
+constexpr int kStampGrid = 256;
 constexpr int kPeelGrid = 1024;
 constexpr int kAdvanceGrid = 64;
 constexpr int kClearGrid = 512;
 
-__device__ __forceinline__ uint32_t decide_round(uint32_t c, unsigned long long alive, const RoundRule& rule,
-                                                 bool allow_compact) {
-  if (c == 0) return kRoundIdle;
+// What the next round does with a frontier of c edges when `alive` edges are
+// live and unstamped.
+__device__ __forceinline__ uint32_t decide_round(uint32_t c, unsigned long long alive, int32_t thr,
+                                                 const RoundRule& rule, bool allow_compact) {
+  if (c == 0) return kRoundMinSup;  // the level is complete
   if (allow_compact && 2ull * alive <= rule.mc) return kRoundCompact;
   if (c == alive) return kRoundEmpty;
-  const bool drop = rule.dense ? double(c) * (double(rule.ws) + rule.tri_cost * double(rule.thr)) >=
+  const bool drop = rule.dense ? double(c) * (double(rule.ws) + rule.tri_cost * double(thr)) >=
                                      double(alive) * double(rule.ws)
                                : double(c) * rule.drop_ratio >= double(alive);
   return drop ? kRoundDrop : kRoundPeel;
 }
 
-__global__ void k_round_start(Ctl* ctl, uint32_t r, unsigned long long alive, RoundRule rule, uint32_t reset) {
+__device__ __forceinline__ void set_action(Ctl* ctl, uint32_t a) {
+  if (a == kRoundMinSup) ctl->minsup = kNone;
+  ctl->action = a;
+}
+
+__global__ void k_round_start(Ctl* ctl, uint32_t r, unsigned long long alive, int32_t thr, RoundRule rule,
+                              uint32_t reset, SearchRule search) {
   if (threadIdx.x != 0) return;
   const uint32_t c = ctl->qcount[0];
   ctl->cur = 0;
   ctl->qcount[1] = 0;
   ctl->round = r;
   ctl->alive = alive;
+  ctl->thr = thr;
   ctl->ticket = 0;
   ctl->stamped = 0;
-  if (reset) ctl->rounds_done = 0;
-  ctl->action = decide_round(c, alive, rule, false);
+  if (reset) {
+    ctl->err &= ~kErrPeelCount;
+    ctl->rounds_done = 0;
+    ctl->levels = 0;
+    ctl->first = 1;
+    ctl->level_kmax = 0;
+    ctl->level_count = 0;
+    ctl->level_floor = -1;
+    ctl->early_ans = search.early_ans;
+    ctl->early_k = search.early_k;
+    ctl->exact_early = search.exact_early;
+    ctl->max_levels = search.max_levels;
+  }
+  set_action(ctl, decide_round(c, alive, thr, rule, false));
 }
 
 // Stamp the current frontier once: state = round, deletion threshold of its
 // base edge = thr, and (walked frontiers of dense cores) its frontier bits.
 // The rounds queued after a drop or an empty level find it stamped already.
-__global__ __launch_bounds__(kBlock) void k_round_stamp(uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t thr,
-                                                        uint32_t* state, const uint32_t* __restrict__ orig,
-                                                        uint32_t* del_thr, const uint32_t* __restrict__ src,
+// At a level's end the same slot reduces the supports of the live edges; the
+// last workgroup records the level (kmax = min support + 2, its edge count
+// and threshold), applies the early exit and moves to threshold min + 1.
+__global__ __launch_bounds__(kBlock) void k_round_stamp(uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t* state,
+                                                        const int32_t* __restrict__ sup,
+                                                        const uint32_t* __restrict__ orig, uint32_t* del_thr,
+                                                        const uint32_t* __restrict__ src,
                                                         const uint32_t* __restrict__ dst,
-                                                        unsigned long long* fbits, uint32_t ws) {
+                                                        unsigned long long* fbits, uint32_t ws, RoundRule rule) {
   const uint32_t a = ctl->action;
+  if (a == kRoundMinSup) {
+    __shared__ uint32_t s_min[kBlock / kWave];
+    uint32_t best = kNone;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < rule.mc; e += gridDim.x * blockDim.x)
+      if (state[e] == kNone) best = min(best, uint32_t(sup[e]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = min(best, uint32_t(__shfl_xor(best, o)));
+    if (lane64() == 0) s_min[threadIdx.x / kWave] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < kBlock / kWave; ++k) best = min(best, s_min[k]);
+      if (best != kNone) atomicMin(&ctl->minsup, best);
+    }
+    if (last_block(&ctl->ticket) && threadIdx.x == 0) {
+      const uint32_t ms = ctl->minsup;
+      const unsigned long long alive = ctl->alive;
+      if (ms == kNone || alive == 0) {  // nothing live: the search is over
+        if ((ms == kNone) != (alive == 0)) ctl->err |= kErrPeelCount;
+        ctl->action = kRoundDone;
+        return;
+      }
+      ctl->level_kmax = ms + 2;
+      ctl->level_count = alive;
+      ctl->level_floor = ctl->thr;
+      ctl->levels += 1;
+      if (ctl->first && ctl->early_ans && alive == ctl->early_ans && (ctl->exact_early || ms + 2 == ctl->early_k)) {
+        ctl->level_kmax = ctl->early_k;
+        ctl->action = kRoundDone;
+        return;
+      }
+      ctl->first = 0;
+      if (ctl->max_levels && ctl->levels >= ctl->max_levels) {  // level budget spent
+        ctl->action = kRoundDone;
+        return;
+      }
+      ctl->thr = int32_t(ms) + 1;
+      ctl->action = (!rule.dense && 2ull * alive <= rule.mc) ? kRoundCompact : kRoundSweep;
+    }
+    return;
+  }
   if (ctl->stamped || (a != kRoundPeel && a != kRoundDrop && a != kRoundEmpty)) return;
   const uint32_t cur = ctl->cur, cnt = ctl->qcount[cur], r = ctl->round;
+  const uint32_t thr = uint32_t(ctl->thr);
   const uint32_t* q = cur ? q1 : q0;
   const bool fb = fbits && a == kRoundPeel;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
@@ -78,6 +149,19 @@ __global__ __launch_bounds__(kBlock) void k_round_stamp(uint32_t* q0, uint32_t* 
   }
 }
 
+// The frontier of a new level: live edges with support below the threshold
+// (the peel kernels run this when the action is kRoundSweep).
+__device__ __forceinline__ void sweep_frontier(uint32_t* q0, uint32_t* q1, Ctl* ctl,
+                                               const uint32_t* __restrict__ state,
+                                               const int32_t* __restrict__ sup, uint32_t mc) {
+  const uint32_t cur = ctl->cur;
+  uint32_t* q = cur ? q1 : q0;
+  const int32_t thr = ctl->thr;
+  tile_compact(
+      mc, [&](uint64_t e) { return state[e] == kNone && sup[e] < thr; },
+      [&](uint32_t at, uint64_t e) { q[at] = uint32_t(e); }, &ctl->qcount[cur]);
+}
+
 __device__ __forceinline__ void clear_edge_bits(uint32_t u, uint32_t v, unsigned long long* bits,
                                                 unsigned long long* fbits, uint32_t ws) {
   const uint64_t a = uint64_t(u) * ws + (v >> 6), b = uint64_t(v) * ws + (u >> 6);
@@ -92,12 +176,22 @@ __device__ __forceinline__ void clear_edge_bits(uint32_t u, uint32_t v, unsigned
 
 // Close a walked round: clear its frontier from the bit matrices, then (last
 // workgroup) make the next queue current and decide what the next round does.
+// After a sweep, only the decision on the swept frontier is taken.
 __global__ __launch_bounds__(kBlock) void k_round_advance(uint32_t* q0, uint32_t* q1, Ctl* ctl,
                                                           const uint32_t* __restrict__ src,
                                                           const uint32_t* __restrict__ dst,
                                                           unsigned long long* bits, unsigned long long* fbits,
                                                           uint32_t ws, RoundRule rule) {
-  if (ctl->action != kRoundPeel || !ctl->stamped) return;
+  const uint32_t a = ctl->action;
+  if (a == kRoundSweep) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const uint32_t c = ctl->qcount[ctl->cur];
+      ctl->stamped = 0;
+      set_action(ctl, decide_round(c, ctl->alive, ctl->thr, rule, false));
+    }
+    return;
+  }
+  if (a != kRoundPeel || !ctl->stamped) return;
   const uint32_t cur = ctl->cur;
   if (bits) {
     const uint32_t cnt = ctl->qcount[cur];
@@ -114,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_round_advance(uint32_t* q0, uint32_t
     ctl->cur = nxt;
     ctl->round += 1;
     ctl->stamped = 0;
-    ctl->action = decide_round(c, ctl->alive, rule, !rule.dense);
+    set_action(ctl, decide_round(c, ctl->alive, ctl->thr, rule, !rule.dense));
   }
 }
 
@@ -150,12 +244,18 @@ __device__ __forceinline__ bool dec_crosses(uint32_t x, int32_t thr, int32_t* su
 // doubled counts (src/kc.cu:122-146).
 template <bool kEidm>
 __global__ __launch_bounds__(kBlock) void k_dense_peel_round(
-    uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, const uint32_t* __restrict__ src,
-    const uint32_t* __restrict__ dst, const unsigned long long* __restrict__ bits,
-    const unsigned long long* __restrict__ fbits, uint32_t ws, const uint32_t* __restrict__ eidm, uint32_t nc,
-    const unsigned long long* __restrict__ bits0, const uint32_t* __restrict__ pref0,
-    const uint32_t* __restrict__ off_u, const uint32_t* __restrict__ eid_u, int32_t* sup) {
-  if (ctl->action != kRoundPeel || !ctl->stamped) return;
+    uint32_t* q0, uint32_t* q1, Ctl* ctl, const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+    const unsigned long long* __restrict__ bits, const unsigned long long* __restrict__ fbits, uint32_t ws,
+    const uint32_t* __restrict__ eidm, uint32_t nc, const unsigned long long* __restrict__ bits0,
+    const uint32_t* __restrict__ pref0, const uint32_t* __restrict__ off_u, const uint32_t* __restrict__ eid_u,
+    const uint32_t* __restrict__ state, int32_t* sup, uint32_t mc) {
+  const uint32_t action = ctl->action;
+  if (action == kRoundSweep) {
+    sweep_frontier(q0, q1, ctl, state, sup, mc);
+    return;
+  }
+  if (action != kRoundPeel || !ctl->stamped) return;
+  const int32_t thr = ctl->thr;
   const uint32_t cur = ctl->cur;
   const uint32_t cnt = ctl->qcount[cur];
   const uint32_t* q = cur ? q1 : q0;
@@ -239,10 +339,16 @@ __global__ __launch_bounds__(kBlock) void k_dense_peel_round(
 // say whether they are in this round's frontier (== round) or gone (< round).
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_sparse_peel_round(
-    uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, const uint32_t* __restrict__ off_u,
-    const uint32_t* __restrict__ adj_u, const uint32_t* __restrict__ eid_u, const uint32_t* __restrict__ src,
-    const uint32_t* __restrict__ dst, const uint32_t* __restrict__ state, int32_t* sup) {
-  if (ctl->action != kRoundPeel || !ctl->stamped) return;
+    uint32_t* q0, uint32_t* q1, Ctl* ctl, const uint32_t* __restrict__ off_u, const uint32_t* __restrict__ adj_u,
+    const uint32_t* __restrict__ eid_u, const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+    const uint32_t* __restrict__ state, int32_t* sup, uint32_t mc) {
+  const uint32_t action = ctl->action;
+  if (action == kRoundSweep) {
+    sweep_frontier(q0, q1, ctl, state, sup, mc);
+    return;
+  }
+  if (action != kRoundPeel || !ctl->stamped) return;
+  const int32_t thr = ctl->thr;
   const uint32_t cur = ctl->cur;
   const uint32_t cnt = ctl->qcount[cur], r = ctl->round;
   const uint32_t* q = cur ? q1 : q0;
@@ -353,65 +459,55 @@ void relabel_offsets(const uint32_t* roff, const uint32_t* vmap, uint32_t n, uin
   k_relabel_offsets<<<grid_for(uint64_t(n) + 1), kBlock, 0, s>>>(roff, vmap, n, off);
 }
 
-void round_start(Ctl* ctl, uint32_t r, uint64_t alive, const RoundRule& rule, bool reset, hipStream_t s) {
-  k_round_start<<<1, kWave, 0, s>>>(ctl, r, alive, rule, reset ? 1u : 0u);
+void round_start(Ctl* ctl, uint32_t r, uint64_t alive, int32_t thr, const RoundRule& rule, bool reset,
+                 const SearchRule& search, hipStream_t s) {
+  k_round_start<<<1, kWave, 0, s>>>(ctl, r, alive, thr, rule, reset ? 1u : 0u, search);
 }
 
-void round_stamp(uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t thr, uint32_t* state, const uint32_t* orig,
-                 uint32_t* del_thr, const uint32_t* src, const uint32_t* dst, uint64_t* fbits, uint32_t ws,
-                 hipStream_t s) {
-  k_round_stamp<<<kStampGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, state, orig, del_thr, src, dst,
-                                              reinterpret_cast<unsigned long long*>(fbits), ws);
+static void launch_stamp(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, hipStream_t s) {
+  k_round_stamp<<<kStampGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.state, a.sup, a.orig, a.del_thr, a.src, a.dst,
+                                              reinterpret_cast<unsigned long long*>(a.fbits), a.ws, rule);
 }
 
-void round_advance(uint32_t* q0, uint32_t* q1, Ctl* ctl, const uint32_t* src, const uint32_t* dst, uint64_t* bits,
-                   uint64_t* fbits, uint32_t ws, const RoundRule& rule, hipStream_t s) {
-  k_round_advance<<<bits ? kAdvanceGrid : 1, kBlock, 0, s>>>(q0, q1, ctl, src, dst,
-                                                            reinterpret_cast<unsigned long long*>(bits),
-                                                            reinterpret_cast<unsigned long long*>(fbits), ws, rule);
-}
+void round_stamp(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, hipStream_t s) { launch_stamp(a, ctl, rule, s); }
 
-void frontier_clear(const uint32_t* q0, const uint32_t* q1, const Ctl* ctl, const uint32_t* src, const uint32_t* dst,
-                    uint64_t* bits, uint64_t* fbits, uint32_t ws, hipStream_t s) {
-  k_frontier_clear<<<kClearGrid, kBlock, 0, s>>>(q0, q1, ctl, src, dst, reinterpret_cast<unsigned long long*>(bits),
-                                                 reinterpret_cast<unsigned long long*>(fbits), ws);
-}
-
-void dense_peel_round(uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, const uint32_t* src, const uint32_t* dst,
-                      const uint64_t* bits, const uint64_t* fbits, uint32_t ws, const uint32_t* eidm, uint32_t nc,
-                      const uint64_t* bits0, const uint32_t* pref0, const uint32_t* off_u, const uint32_t* eid_u,
-                      int32_t* sup, hipStream_t s) {
-  auto b = reinterpret_cast<const unsigned long long*>(bits);
-  auto f = reinterpret_cast<const unsigned long long*>(fbits);
-  auto b0 = reinterpret_cast<const unsigned long long*>(bits0);
-  if (eidm)
-    k_dense_peel_round<true><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, src, dst, b, f, ws, eidm, nc, b0, pref0,
-                                                          off_u, eid_u, sup);
-  else
-    k_dense_peel_round<false><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, src, dst, b, f, ws, eidm, nc, b0, pref0,
-                                                           off_u, eid_u, sup);
-}
-
-void sparse_peel_round(uint32_t* q0, uint32_t* q1, Ctl* ctl, int32_t thr, int lanes, const uint32_t* off_u,
-                       const uint32_t* adj_u, const uint32_t* eid_u, const uint32_t* src, const uint32_t* dst,
-                       const uint32_t* state, int32_t* sup, hipStream_t s) {
-  switch (lanes) {
-    case 8:
-      k_sparse_peel_round<8><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, off_u, adj_u, eid_u, src, dst, state, sup);
-      break;
-    case 16:
-      k_sparse_peel_round<16><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, off_u, adj_u, eid_u, src, dst, state,
-                                                           sup);
-      break;
-    case 32:
-      k_sparse_peel_round<32><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, off_u, adj_u, eid_u, src, dst, state,
-                                                           sup);
-      break;
-    default:
-      k_sparse_peel_round<64><<<kPeelGrid, kBlock, 0, s>>>(q0, q1, ctl, thr, off_u, adj_u, eid_u, src, dst, state,
-                                                           sup);
-      break;
+void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, int count, hipStream_t s) {
+  auto bits = reinterpret_cast<unsigned long long*>(a.bits);
+  auto fbits = reinterpret_cast<unsigned long long*>(a.fbits);
+  auto b0 = reinterpret_cast<const unsigned long long*>(a.bits0);
+  for (int i = 0; i < count; ++i) {
+    if (rule.dense) {
+      if (a.eidm)
+        k_dense_peel_round<true><<<kPeelGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst, bits, fbits, a.ws, a.eidm,
+                                                              a.nc, b0, a.pref0, a.off_u, a.eid_u, a.state, a.sup,
+                                                              a.mc);
+      else
+        k_dense_peel_round<false><<<kPeelGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst, bits, fbits, a.ws,
+                                                               a.eidm, a.nc, b0, a.pref0, a.off_u, a.eid_u, a.state,
+                                                               a.sup, a.mc);
+    } else {
+#define KT_SPARSE_ROUND(G)                                                                                 \
+  k_sparse_peel_round<G><<<kPeelGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.off_u, a.adj_u, a.eid_u, a.src, a.dst, \
+                                                      a.state, a.sup, a.mc)
+      switch (lanes) {
+        case 8: KT_SPARSE_ROUND(8); break;
+        case 16: KT_SPARSE_ROUND(16); break;
+        case 32: KT_SPARSE_ROUND(32); break;
+        default: KT_SPARSE_ROUND(64); break;
+      }
+#undef KT_SPARSE_ROUND
+    }
+    k_round_advance<<<rule.dense ? kAdvanceGrid : 1, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst,
+                                                                     rule.dense ? bits : nullptr,
+                                                                     rule.dense ? fbits : nullptr, a.ws, rule);
+    launch_stamp(a, ctl, rule, s);
   }
+}
+
+void frontier_clear(const PeelArgs& a, const Ctl* ctl, hipStream_t s) {
+  k_frontier_clear<<<kClearGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst,
+                                                 reinterpret_cast<unsigned long long*>(a.bits),
+                                                 reinterpret_cast<unsigned long long*>(a.fbits), a.ws);
 }
 
 void dense_eidm(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, uint32_t nc, uint32_t* eidm,

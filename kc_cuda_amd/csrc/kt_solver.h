@@ -62,6 +62,8 @@ struct CoreView {
 struct TrussOutcome {
   uint32_t kmax = 0;
   uint64_t count = 0;
+  uint64_t alive = 0;      // live edges when the search stopped
+  bool cut_short = false;  // stopped by the level budget with live edges left
   int32_t floor_thr = -1;  // survivors: del_thr == kNone or del_thr > floor_thr
   uint32_t levels = 0;
   uint64_t rounds = 0;
@@ -92,6 +94,11 @@ class Solver {
   void set_force_edge_count(bool v) { force_edge_count_ = v; }
   void set_force_atomic_build(bool v) { force_atomic_build_ = v; }
   void set_core_mode(int m) { core_mode_ = m; }  // 0 auto, 1 sparse, 2 dense
+  // pass 1 on dense views: walk `walk_levels` levels, then probe thresholds
+  void set_gallop(bool v, uint32_t walk_levels) {
+    gallop_ = v;
+    walk_levels_ = walk_levels ? walk_levels : 1;
+  }
 
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }
@@ -120,6 +127,9 @@ class Solver {
   // whatever the minimum support (the reference's rule, src/kc.cu:1240).
   TrussOutcome truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k,
                             bool exact_early = false);
+  // Dense views only: the same answer as truss_search(cv, 1, 0, 0) found by
+  // probing thresholds with backups instead of walking every level.
+  TrussOutcome truss_gallop(CoreView* cv);
 
   // reference-compatible stage entry points (kt_compat.hip)
   void vertex_state_from_host(const uint32_t* deg_origin, const uint32_t* mask);
@@ -147,10 +157,13 @@ class Solver {
   uint32_t read_u32(const uint32_t* dptr);
   uint64_t read_u64(const uint64_t* dptr);
   Ctl read_ctl();
-  RoundRule round_rule(const CoreView& cv, int32_t thr) const;
+  RoundRule round_rule(const CoreView& cv) const;
+  PeelArgs peel_args(const CoreView& cv);
   // frontier of the live edges below thr, stamped; the rounds' starting state
-  void start_rounds(CoreView* cv, int32_t thr, uint32_t r, uint64_t alive, bool reset);
-  void enqueue_rounds(CoreView* cv, int32_t thr, int count);
+  // (reset: a new search with its early-exit rule)
+  void start_rounds(CoreView* cv, int32_t thr, uint32_t r, uint64_t alive, bool reset, const SearchRule& search);
+  TrussOutcome run_levels(CoreView* cv, int32_t thr, uint64_t alive, uint32_t* r_io, const SearchRule& search,
+                          bool allow_redensify);
   void zero_u32(uint32_t* dptr);
   void exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, uint32_t n_plus_1);
   void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n_plus_1);
@@ -168,9 +181,12 @@ class Solver {
   bool force_atomic_build_ = false;
   int core_mode_ = 0;
   DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_;
+  DevBuf bk_state_, bk_sup_, bk_bits_, bk_del_;  // threshold-probe backups
   uint64_t eidm_budget_ = uint64_t(4) << 30;  // bytes for the edge-id matrix
   double dense_tri_cost_ = 8.0, sparse_drop_ratio_ = 3.0;
   bool trace_ = false;  // KT_TRACE=1: per-round frontier sizes on stderr
+  bool gallop_ = true;
+  uint32_t walk_levels_ = 16;
   void make_dense(CoreView* cv);
   HeavyList heavy() {
     heavy_.ensure(sizeof(uint32_t) * 2 * (size_t(n_) + 2));

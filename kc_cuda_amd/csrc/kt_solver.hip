@@ -57,7 +57,7 @@ Solver::~Solver() {
                    &adj_d_, &src_, &din_, &din_off_, &tkeys_, &tvals_, &tiota_, &in_src_, &off_u_,
                    &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &del_thr_, &orig0_,
                    &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &fbits_, &pref0_, &eidm_, &heavy_,
-                   &vflag_, &vmap_, &roff_};
+                   &vflag_, &vmap_, &roff_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_};
   for (auto* b : all) b->release();
   for (auto& w : work_) {
     w.off.release();

@@ -392,128 +392,209 @@ bool Solver::try_redensify(CoreView* cv) {
 // skipped by jumping to the minimum live support. With early_ans != 0, a
 // first level of exactly early_ans edges at kmax early_k ends the search
 // (src/kc.cu:1240-1241).
-RoundRule Solver::round_rule(const CoreView& cv, int32_t thr) const {
+RoundRule Solver::round_rule(const CoreView& cv) const {
   RoundRule rule{};
   rule.dense = cv.dense ? 1u : 0u;
   rule.ws = cv.ws;
   rule.mc = cv.mc;
-  rule.thr = thr;
   rule.tri_cost = dense_tri_cost_;
   rule.drop_ratio = sparse_drop_ratio_;
   return rule;
 }
 
-void Solver::start_rounds(CoreView* cv, int32_t thr, uint32_t r, uint64_t alive, bool reset) {
-  uint32_t* q0 = eq0_.as<uint32_t>();
-  zero_u32(&ctl_->qcount[0]);
-  edge_frontier_init(cv->state, cv->sup, cv->mc, thr, q0, &ctl_->qcount[0], stream_);
-  round_start(ctl_, r, alive, round_rule(*cv, thr), reset, stream_);
-  round_stamp(q0, eq1_.as<uint32_t>(), ctl_, uint32_t(thr), cv->state, cv->orig, cv->del_thr, cv->src, cv->adj_d,
-              cv->dense ? cv->fbits : nullptr, cv->ws, stream_);
-}
-
-void Solver::enqueue_rounds(CoreView* cv, int32_t thr, int count) {
-  uint32_t* q0 = eq0_.as<uint32_t>();
-  uint32_t* q1 = eq1_.as<uint32_t>();
-  const RoundRule rule = round_rule(*cv, thr);
-  const double avg = cv->nc ? double(2ull * cv->mc) / cv->nc : 0.0;
-  const int lanes = lanes_peel_ ? lanes_peel_ : auto_lanes(avg);
-  for (int i = 0; i < count; ++i) {
-    if (cv->dense)
-      dense_peel_round(q0, q1, ctl_, thr, cv->src, cv->adj_d, cv->bits, cv->fbits, cv->ws, cv->eidm, cv->nc,
-                       cv->bits0, cv->pref0, cv->off_u, cv->eid_u, cv->sup, stream_);
-    else
-      sparse_peel_round(q0, q1, ctl_, thr, lanes, cv->off_u, cv->adj_u, cv->eid_u, cv->src, cv->adj_d, cv->state,
-                        cv->sup, stream_);
-    round_advance(q0, q1, ctl_, cv->src, cv->adj_d, cv->dense ? cv->bits : nullptr,
-                  cv->dense ? cv->fbits : nullptr, cv->ws, rule, stream_);
-    round_stamp(q0, q1, ctl_, uint32_t(thr), cv->state, cv->orig, cv->del_thr, cv->src, cv->adj_d,
-                cv->dense ? cv->fbits : nullptr, cv->ws, stream_);
+PeelArgs Solver::peel_args(const CoreView& cv) {
+  PeelArgs a{};
+  a.q0 = eq0_.as<uint32_t>();
+  a.q1 = eq1_.as<uint32_t>();
+  a.src = cv.src;
+  a.dst = cv.adj_d;
+  a.orig = cv.orig;
+  a.state = cv.state;
+  a.del_thr = cv.del_thr;
+  a.sup = cv.sup;
+  a.mc = cv.mc;
+  a.nc = cv.nc;
+  a.ws = cv.ws;
+  a.off_u = cv.off_u;
+  a.adj_u = cv.adj_u;
+  a.eid_u = cv.eid_u;
+  if (cv.dense) {
+    a.bits = cv.bits;
+    a.fbits = cv.fbits;
+    a.bits0 = cv.bits0;
+    a.pref0 = cv.pref0;
+    a.eidm = cv.eidm;
   }
+  return a;
 }
 
+void Solver::start_rounds(CoreView* cv, int32_t thr, uint32_t r, uint64_t alive, bool reset,
+                          const SearchRule& search) {
+  const PeelArgs a = peel_args(*cv);
+  zero_u32(&ctl_->qcount[0]);
+  edge_frontier_init(cv->state, cv->sup, cv->mc, thr, a.q0, &ctl_->qcount[0], stream_);
+  const RoundRule rule = round_rule(*cv);
+  round_start(ctl_, r, alive, thr, rule, reset, search, stream_);
+  round_stamp(a, ctl_, rule, stream_);
+}
+
+// Truss levels on the core view, upward from threshold thr0 (an edge stays
+// while its support >= thr, i.e. in the (thr+2)-truss). Each completed level
+// moves the threshold to its minimum live support + 1, so empty levels are
+// skipped. With early_ans != 0, a first level of exactly early_ans edges at
+// kmax early_k (any kmax when exact_early) ends the search (src/kc.cu:1240-1241).
+// Rounds and level changes run on the device; the host reads the control
+// block between batches and handles dropped frontiers and compactions.
 TrussOutcome Solver::truss_search(CoreView* cv, int32_t thr0, uint64_t early_ans, uint32_t early_k,
                                   bool exact_early) {
+  if (cv->mc == 0) return TrussOutcome{};
+  SearchRule search{};
+  search.early_ans = early_ans;
+  search.early_k = early_k;
+  search.exact_early = exact_early ? 1u : 0u;
+  uint32_t r = 1;
+  return run_levels(cv, thr0, cv->mc, &r, search, true);
+}
+
+// Runs levels from threshold thr with `alive` live edges on the device until
+// the search rule ends it (or the level empties); handles dropped frontiers
+// and compactions. *r is the next round stamp on entry and on exit.
+TrussOutcome Solver::run_levels(CoreView* cv, int32_t thr, uint64_t alive, uint32_t* r_io, const SearchRule& search,
+                                bool allow_redensify) {
   constexpr int kMaxBatch = 32;
   TrussOutcome out;
-  if (cv->mc == 0) return out;
-  uint64_t alive = cv->mc;
-  uint32_t r = 1;
-  int32_t thr = thr0;
-  bool first = true, reset = true;
+  uint32_t r = *r_io;
+  start_rounds(cv, thr, r, alive, true, search);
+  int batch = 2;
+  uint64_t queued = 0;  // rounds since the last restart; each removes an edge or changes level
   for (;;) {
-    if (!cv->dense && 2 * alive <= cv->mc) {  // half the edges are dead: compact, keep supports
-      compact_core(cv, true);
-      ++out.compactions;
-      r = 1;
+    if (queued > 3 * uint64_t(cv->mc) + 4 * kMaxBatch) fail("edge peel did not converge");
+    const double avg = cv->nc ? double(2ull * cv->mc) / cv->nc : 0.0;
+    peel_rounds(peel_args(*cv), ctl_, round_rule(*cv), lanes_peel_ ? lanes_peel_ : auto_lanes(avg), batch, stream_);
+    queued += batch;
+    const Ctl h = read_ctl();
+    if (trace_)
+      std::fprintf(stderr, "[kt] edge-peel batch=%d action=%u thr=%d alive=%llu mc=%u frontier=%u levels=%u\n",
+                   batch, h.action, h.thr, h.alive, cv->mc, h.qcount[h.cur], h.levels);
+    if (h.err & kErrPeelCount) fail("edge peel bookkeeping mismatch (live count)");
+    out.rounds = h.rounds_done;
+    out.levels = h.levels;
+    out.kmax = h.level_kmax;
+    out.count = h.level_count;
+    out.floor_thr = h.level_floor;
+    if (h.action == kRoundPeel || h.action == kRoundMinSup || h.action == kRoundSweep) {
+      batch = std::min(batch * 2, kMaxBatch);
+      continue;
     }
-    start_rounds(cv, thr, r, alive, reset);
-    reset = false;
-    // Rounds run on the device in batches; the host only looks at the
-    // control block between batches and acts when a round stops walking.
-    int batch = 2;
-    uint64_t queued = 0;  // every walked round removes an edge
-    for (;;) {
-      if (queued > uint64_t(cv->mc) + 2 * kMaxBatch) fail("edge peel did not converge");
-      enqueue_rounds(cv, thr, batch);
-      queued += batch;
-      const Ctl h = read_ctl();
-      if (trace_)
-        std::fprintf(stderr, "[kt] edge-peel thr=%d batch=%d action=%u alive=%llu mc=%u frontier=%u round=%u\n", thr,
-                     batch, h.action, h.alive, cv->mc, h.qcount[h.cur], h.round);
-      if (h.action == kRoundPeel) {
-        batch = std::min(batch * 2, kMaxBatch);
-        continue;
-      }
-      out.rounds = h.rounds_done;
-      alive = h.alive;
-      r = h.round + 1;
-      if (h.action == kRoundIdle) break;
-      if (h.action == kRoundEmpty) {  // every live edge went: the level is empty
-        alive = 0;
-        break;
-      }
-      if (h.action == kRoundDrop) {  // drop the frontier, recount the rest exactly
-        if (cv->dense) {
-          if (try_redensify(cv)) {  // smaller matrix on the vertices left, exact recount
-            if (cv->mc != alive) fail("edge peel bookkeeping mismatch (redensify)");
-            r = 1;
-          } else {
-            frontier_clear(eq0_.as<uint32_t>(), eq1_.as<uint32_t>(), ctl_, cv->src, cv->adj_d, cv->bits,
-                           cv->fbits, cv->ws, stream_);
-            dense_support(cv->src, cv->adj_d, cv->mc, cv->bits, cv->ws, cv->state, cv->sup, stream_);
-          }
-        } else {
-          compact_core(cv, false);
-          alive = cv->mc;
-          r = 1;
-        }
-      } else {  // half the edges are dead: compact before stamping the next frontier
-        compact_core(cv, true);
-        if (cv->mc != alive) fail("edge peel bookkeeping mismatch");
-        r = 1;
-      }
-      ++out.compactions;
-      start_rounds(cv, thr, r, alive, false);
-      batch = 2;
-      queued = 0;
-    }
-    if (alive == 0) break;
-    if (alive > cv->mc) fail("edge peel bookkeeping mismatch (live count)");
-    fill_u32(&ctl_->minsup, 1, kNone, stream_);
-    edge_min_support(cv->state, cv->sup, cv->mc, &ctl_->minsup, stream_);
-    const uint32_t ms = read_u32(&ctl_->minsup);
-    if (ms == kNone || ms >= 0x7FFFFFFFu) fail("edge peel bookkeeping mismatch (no live edge)");
-    out.kmax = ms + 2;
-    out.count = alive;
-    out.floor_thr = thr;
-    ++out.levels;
-    if (first && early_ans && alive == early_ans && (exact_early || out.kmax == early_k)) {
-      out.kmax = early_k;
+    out.alive = h.alive;
+    r = h.round + 1;
+    if (h.action == kRoundDone || h.action == kRoundEmpty) {
+      out.cut_short = h.action == kRoundDone && h.alive > 0 && search.max_levels && h.levels >= search.max_levels;
       break;
     }
-    first = false;
-    thr = int32_t(ms) + 1;
+    alive = h.alive;
+    thr = h.thr;
+    if (h.action == kRoundDrop) {  // drop the stamped frontier, recount the rest exactly
+      if (cv->dense) {
+        if (allow_redensify && try_redensify(cv)) {  // smaller matrix on the vertices left, exact recount
+          if (cv->mc != alive) fail("edge peel bookkeeping mismatch (redensify)");
+          r = 1;
+        } else {
+          frontier_clear(peel_args(*cv), ctl_, stream_);
+          dense_support(cv->src, cv->adj_d, cv->mc, cv->bits, cv->ws, cv->state, cv->sup, stream_);
+        }
+      } else {
+        compact_core(cv, false);
+        alive = cv->mc;
+        r = 1;
+      }
+    } else if (h.action == kRoundCompact) {  // half the edges are dead: compact, keep supports
+      compact_core(cv, true);
+      if (cv->mc != alive) fail("edge peel bookkeeping mismatch (compaction)");
+      r = 1;
+    } else {
+      fail("edge peel: unexpected round action");
+    }
+    ++out.compactions;
+    start_rounds(cv, thr, r, alive, false, search);
+    batch = 2;
+    queued = 0;
+  }
+  *r_io = r;
+  return out;
+}
+
+// The largest threshold t whose level is non-empty, on a dense view. The
+// first walk_levels_ levels are walked as truss_search does; if the search is
+// not over by then, thresholds are probed instead: each probe peels one
+// level at t from the last non-empty state (kept in a backup) and is undone
+// when it empties the view. Thresholds gallop up by doubling steps from the
+// minimum support, then bisect. The result equals the level walk's: the last
+// non-empty probe leaves the t-level, whose minimum support is t* (a level at
+// t* + 1 is empty), so kmax = t* + 2 with its edge count, and every edge
+// deleted on the way carries a threshold <= the floor. (The reference's
+// speculative k-steps with backups, src/kc.cu:1189-1196, 1252-1272.)
+TrussOutcome Solver::truss_gallop(CoreView* cv) {
+  TrussOutcome out;
+  if (cv->mc == 0) return out;
+  SearchRule walk{};
+  walk.max_levels = walk_levels_;
+  uint32_t r = 1;
+  TrussOutcome lv = run_levels(cv, 1, cv->mc, &r, walk, true);
+  if (!lv.cut_short) return lv;  // the walk finished the search
+  out.rounds += lv.rounds;
+  out.levels += lv.levels;
+  out.compactions += lv.compactions;
+  out.kmax = lv.kmax;
+  out.count = lv.count;
+  out.floor_thr = lv.floor_thr;
+  const size_t mc = cv->mc, words = size_t(cv->nc) * cv->ws;
+  bk_state_.ensure(sizeof(uint32_t) * std::max<size_t>(mc, 1));
+  bk_sup_.ensure(sizeof(int32_t) * std::max<size_t>(mc, 1));
+  bk_bits_.ensure(sizeof(uint64_t) * std::max<size_t>(words, 1));
+  bk_del_.ensure(sizeof(uint32_t) * std::max<size_t>(cv->mc0, 1));
+  auto copy = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes) KT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_));
+  };
+  auto save = [&]() {
+    copy(bk_state_.p, cv->state, sizeof(uint32_t) * mc);
+    copy(bk_sup_.p, cv->sup, sizeof(int32_t) * mc);
+    copy(bk_bits_.p, cv->bits, sizeof(uint64_t) * words);
+    copy(bk_del_.p, cv->del_thr, sizeof(uint32_t) * cv->mc0);
+  };
+  auto restore = [&]() {
+    copy(cv->state, bk_state_.p, sizeof(uint32_t) * mc);
+    copy(cv->sup, bk_sup_.p, sizeof(int32_t) * mc);
+    copy(cv->bits, bk_bits_.p, sizeof(uint64_t) * words);
+    copy(cv->del_thr, bk_del_.p, sizeof(uint32_t) * cv->mc0);
+  };
+  SearchRule single{};
+  single.max_levels = 1;
+  int64_t lo = int64_t(out.kmax) - 2;  // non-empty: the current state is its level
+  int64_t hi = -1;                      // smallest known empty threshold
+  uint64_t alive = out.count;
+  int64_t step = 1;
+  while (hi < 0 || hi - lo > 1) {
+    const int64_t t = hi < 0 ? lo + step : lo + (hi - lo) / 2;
+    save();
+    const TrussOutcome p = run_levels(cv, int32_t(t), alive, &r, single, false);
+    out.rounds += p.rounds;
+    out.levels += 1;
+    out.compactions += p.compactions;
+    if (p.alive > 0 && p.kmax >= 2) {
+      lo = int64_t(p.kmax) - 2;  // >= t
+      alive = p.alive;
+      out.kmax = p.kmax;
+      out.count = p.count;
+      out.floor_thr = p.floor_thr;
+      step *= 2;
+    } else {
+      restore();
+      hi = t;
+    }
+    if (trace_)
+      std::fprintf(stderr, "[kt] gallop probe t=%lld -> %s lo=%lld hi=%lld alive=%llu\n", (long long)t,
+                   p.alive ? "kept" : "empty", (long long)lo, (long long)hi, (unsigned long long)alive);
   }
   return out;
 }
@@ -561,7 +642,7 @@ void Solver::solve_csr(kt_result* res) {
   CoreView cv;
   extract_core(*levels_[depth_ - 1], &cv);
   count_support(&cv, nullptr);
-  TrussOutcome p1 = truss_search(&cv, 1, 0, 0);
+  TrussOutcome p1 = cv.dense && gallop_ ? truss_gallop(&cv) : truss_search(&cv, 1, 0, 0);
   res->k_hat = p1.kmax;
   res->k_hat_edges = p1.count;
   KT_HIP(hipEventRecord(ev_[3], stream_));

@@ -36,21 +36,70 @@ inline uint64_t stream_key(uint64_t seed, uint64_t stream) {
 inline double u01(uint64_t r) { return double(r >> 11) * (1.0 / 9007199254740992.0); }
 
 // Sort + unique canonical keys (u << 32 | v, u < v) and unpack to pairs.
+// Keys are scattered into 2^16 buckets by their top bits (per-thread
+// histograms, so the scatter is stable and deterministic), each bucket is
+// sorted and de-duplicated on its own, and the buckets are concatenated.
 int64_t finish(std::vector<uint64_t>& keys, uint32_t** out) {
-  __gnu_parallel::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  // all-ones keys mark dropped samples (self-loops); after sort+unique at most one, last
-  if (!keys.empty() && keys.back() == ~uint64_t(0)) keys.pop_back();
-  const int64_t m = int64_t(keys.size());
-  uint32_t* p = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * 2 * size_t(m > 0 ? m : 1)));
+  constexpr int kBits = 16;
+  constexpr size_t kB = size_t(1) << kBits;
+  const size_t L = keys.size();
+  const int nt = omp_get_max_threads();
+  std::vector<uint64_t> tmp(L);
+  std::vector<size_t> cnt(size_t(nt) * kB, 0), start(kB + 1, 0);
+  auto bucket = [](uint64_t k) { return size_t(k >> (64 - kBits)); };
+#pragma omp parallel num_threads(nt)
+  {
+    const int t = omp_get_thread_num();
+    const size_t lo = L * t / nt, hi = L * (t + 1) / nt;
+    size_t* c = &cnt[size_t(t) * kB];
+    for (size_t i = lo; i < hi; ++i) ++c[bucket(keys[i])];
+#pragma omp barrier
+#pragma omp single
+    {
+      size_t run = 0;
+      for (size_t b = 0; b < kB; ++b) {
+        start[b] = run;
+        for (int s = 0; s < nt; ++s) {
+          const size_t v = cnt[size_t(s) * kB + b];
+          cnt[size_t(s) * kB + b] = run;
+          run += v;
+        }
+      }
+      start[kB] = run;
+    }
+    for (size_t i = lo; i < hi; ++i) tmp[c[bucket(keys[i])]++] = keys[i];
+  }
+  std::vector<size_t> kept(kB + 1, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t b = 0; b < int64_t(kB); ++b) {
+    auto first = tmp.begin() + start[b], last = tmp.begin() + start[b + 1];
+    std::sort(first, last);
+    kept[b] = size_t(std::unique(first, last) - first);
+  }
+  size_t m = 0;
+  for (size_t b = 0; b < kB; ++b) {
+    const size_t k = kept[b];
+    kept[b] = m;
+    m += k;
+  }
+  // all-ones keys mark dropped samples (self-loops): after de-duplication at
+  // most one is left, as the last key of the last bucket
+  const size_t last_kept = m - kept[kB - 1];
+  if (last_kept && tmp[start[kB - 1] + last_kept - 1] == ~uint64_t(0)) --m;
+  std::vector<uint64_t>().swap(keys);
+  uint32_t* p = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * 2 * (m > 0 ? m : 1)));
   if (!p) return -1;
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < m; ++i) {
-    p[2 * i] = uint32_t(keys[i] >> 32);
-    p[2 * i + 1] = uint32_t(keys[i]);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t b = 0; b < int64_t(kB); ++b) {
+    const size_t o = kept[b], n = (b + 1 < int64_t(kB) ? kept[b + 1] : m) - o;
+    for (size_t i = 0; i < n; ++i) {
+      const uint64_t k = tmp[start[b] + i];
+      p[2 * (o + i)] = uint32_t(k >> 32);
+      p[2 * (o + i) + 1] = uint32_t(k);
+    }
   }
   *out = p;
-  return m;
+  return int64_t(m);
 }
 
 inline uint64_t canon(uint32_t a, uint32_t b) {
@@ -178,8 +227,10 @@ int64_t ktgen_rmat(int scale, int edge_factor, double a, double b, double c, int
 // Chung-Lu power-law graph: weights w_i = (i+1)^(-1/(gamma-1)), scaled so the
 // expected degree sum is 2 * samples, capped at sqrt(sum w); `samples`
 // endpoint pairs drawn independently proportional to weight (Walker/Vose
-// alias table), self-loops dropped, deduplicated, plus a planted clique.
-int64_t ktgen_chung_lu(uint32_t n, double gamma, int64_t samples, uint32_t clique, uint64_t seed,
+// alias table), optionally relabelled by a random permutation (hubs are not
+// left at the small ids), self-loops dropped, deduplicated, plus a planted
+// clique.
+int64_t ktgen_chung_lu(uint32_t n, double gamma, int64_t samples, int relabel, uint32_t clique, uint64_t seed,
                        uint32_t** out) {
   if (n < 2 || samples <= 0) {
     std::vector<uint64_t> none;
@@ -232,6 +283,8 @@ int64_t ktgen_chung_lu(uint32_t n, double gamma, int64_t samples, uint32_t cliqu
   }
   std::vector<double>().swap(w);
   const uint64_t key = stream_key(seed, 1);
+  std::vector<uint32_t> perm;
+  if (relabel) perm = permutation(n, stream_key(seed, 3));
   std::vector<uint64_t> keys(static_cast<size_t>(samples));
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < samples; ++i) {
@@ -241,6 +294,7 @@ int64_t ktgen_chung_lu(uint32_t n, double gamma, int64_t samples, uint32_t cliqu
       uint32_t col = uint32_t((r >> 32) * uint64_t(n) >> 32);
       float f = float(uint32_t(r) >> 8) * (1.0f / 16777216.0f);
       e[s] = (f < prob[col]) ? col : alias[col];
+      if (relabel) e[s] = perm[e[s]];
     }
     keys[size_t(i)] = (e[0] == e[1]) ? ~uint64_t(0) : canon(e[0], e[1]);
   }

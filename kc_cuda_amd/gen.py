@@ -36,7 +36,7 @@ def _gen():
                                    ctypes.c_double, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
                                    _u32pp]
         lib.ktgen_chung_lu.restype = ctypes.c_int64
-        lib.ktgen_chung_lu.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_int64,
+        lib.ktgen_chung_lu.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_int64, ctypes.c_int,
                                        ctypes.c_uint32, ctypes.c_uint64, _u32pp]
         lib.ktgen_write_tsv.restype = ctypes.c_int64
         lib.ktgen_write_tsv.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]
@@ -73,11 +73,11 @@ def rmat(scale: int = 22, edge_factor: int = 16, a: float = 0.57, b: float = 0.1
     return _take(lib, m, p), 1 << scale
 
 
-def chung_lu(n: int = 50_000_000, gamma: float = 2.1, samples: int = 560_000_000,
-             clique: int = 60, seed: int = 2):
+def chung_lu(n: int = 50_000_000, gamma: float = 2.1, samples: int = 505_000_000,
+             clique: int = 60, seed: int = 2, relabel: bool = True):
     lib = _gen()
     p = ctypes.POINTER(ctypes.c_uint32)()
-    m = lib.ktgen_chung_lu(n, gamma, samples, clique, seed, ctypes.byref(p))
+    m = lib.ktgen_chung_lu(n, gamma, samples, int(relabel), clique, seed, ctypes.byref(p))
     return _take(lib, m, p), n
 
 
@@ -98,7 +98,7 @@ def workload(name: str):
     if name == "rmat22":
         return rmat(22, 16, 0.57, 0.19, 0.19, True, 0, 1)
     if name == "chung_lu_50m":
-        return chung_lu(50_000_000, 2.1, 560_000_000, 60, 2)
+        return chung_lu(50_000_000, 2.1, 505_000_000, 60, 2)
     if name.startswith("rmat"):
         scale = int(name[4:])
         return rmat(scale, 16, 0.57, 0.19, 0.19, True, 0, 1)

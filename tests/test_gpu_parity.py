@@ -72,10 +72,12 @@ def test_kernel_variants_agree(ctx, lanes, force_edge):
         ctx.tune(0, 0, False)
 
 
-@pytest.mark.parametrize("core", ["sparse", "dense"])
+@pytest.mark.parametrize("core", ["sparse", "dense", "dense-walk", "dense-probe"])
 @pytest.mark.parametrize("seed", range(8))
 def test_core_representations_agree(ctx, core, seed):
-    """Sorted-row and bit-matrix cores give the same kmax-truss."""
+    """Sorted-row and bit-matrix cores give the same kmax-truss, and on
+    bit-matrix cores the first pass's threshold probes give the same k-hat as
+    walking every level."""
     from kc_cuda_amd import gen
     O = _oracle()
     if seed < 4:
@@ -87,10 +89,11 @@ def test_core_representations_agree(ctx, core, seed):
     print(f"[gpu] core={core} seed={seed} m={len(pairs)}", flush=True)
     ref = O.solve_pairs(pairs, n, want_edges=True)
     try:
-        ctx.tune(core=core)
+        core_kind, _, pass1 = core.partition("-")
+        ctx.tune(core=core_kind, pass1=pass1 or "auto")
         r = ctx.solve_pairs(pairs, n)
         assert (r.kmax, r.truss_edges) == (ref.kmax, ref.truss_edges)
-        assert r.k_hat == ref.k_hat
+        assert (r.k_hat, r.k_hat_edges) == (ref.k_hat, ref.k_hat_edges)
         np.testing.assert_array_equal(ctx.truss_edges(), canon_pairs(ref.truss))
     finally:
         ctx.tune()
