@@ -11,9 +11,13 @@ self-loops dropped, randomly relabelled, seed 1 (64,152,088 edges).
 
 Prints one JSON line on stdout (everything else goes to stderr):
   value        edges/s = m / wall time per solve
-  roofline     the support-count kernel (the dominant kernel): algorithmic
-               bytes 4*W(C+) + 4*m_C + 4*(n_C+1) per launch over its HIP-event
-               duration, against 8 TB/s HBM3E
+  roofline     the pass-2 support-count kernel (the count the reference
+               spends most of its device time on): algorithmic bytes
+               4*W(C+) + 4*m_C + 4*(n_C+1) per launch (one exact merge over
+               every oriented row pair, support write, offsets) over its
+               HIP-event duration, against 8 TB/s HBM3E; `traffic` is the
+               rocprofv3-measured HBM bytes of the same kernel when a summary
+               for this workload is committed under profiles/
   cpu_baseline the reference's kc_cpu (compiled from its sources into
                oracle/_ref) on the same graph written as TSV, timed like the
                reference's own "All time cost - Reading file Time Cost";
@@ -36,7 +40,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "kmax-truss wall time & edges/s (1 GPU); batched graphs/s; p99 serve latency"
-PROFILE_TRAFFIC = ROOT / "profiles" / "support_bitmap_rmat22_traffic.json"
+PROFILES = ROOT / "profiles"
+COUNT_KERNELS = {0: "k_support_bitmap", 1: "k_support_edges", 2: "k_dense_support"}
 
 
 def log(*a):
@@ -93,6 +98,9 @@ def main() -> int:
     ap.add_argument("--workload", default="rmat22")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=120.0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after timing, one device-to-device copy of this size (HBM counter "
+                         "calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -128,15 +136,24 @@ def main() -> int:
                         "ms_peel", "ms_count_kernel")}
     log(f"[bench] timed {args.steps} steps: {wall * 1e3:.2f} ms/step; stages {stages}")
     model = byte_model(n, m, st)
+    kernel = COUNT_KERNELS.get(st.count_kernel, "k_support_edges")
     count_bytes = 4 * st.w_core + 4 * st.m_core + 4 * (st.n_core + 1)
     count_s = stages["ms_count_kernel"] / 1e3
     achieved = count_bytes / count_s / 1e9 if count_s > 0 else 0.0
     traffic = None
-    if PROFILE_TRAFFIC.exists():
+    tfile = PROFILES / f"{kernel}_{args.workload}_traffic.json"
+    if tfile.exists():
         try:
-            traffic = json.loads(PROFILE_TRAFFIC.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
             traffic = None
+    if args.calibrate_copy_mib > 0:
+        nb = args.calibrate_copy_mib << 20
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy_device(b, a, nb)
+        log(f"[bench] calibration copy: {nb} bytes device to device")
+        a.free()
+        b.free()
     d.free()
     ctx.close()
 
@@ -171,7 +188,7 @@ def main() -> int:
                    "w_peeled": int(st.w_peeled), "stages_ms": stages,
                    "solve_bytes_model": model["total"],
                    "solve_roofline_frac": model["total"] / (wall * HBM_PEAK_GBS * 1e9)},
-        "roofline": {"kernel": "k_support_bitmap", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": kernel, "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": count_bytes},
         "cpu_baseline": base,

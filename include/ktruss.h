@@ -66,6 +66,8 @@ typedef struct kt_stats {
   uint64_t w_core;      /* W(C+) */
   uint64_t w_peeled;    /* W(C+ \ T+) */
   uint64_t truss_edges;
+  uint32_t count_kernel;  /* pass-2 support count: 0 LDS bitmap, 1 per edge, 2 bit matrix */
+  uint32_t core_words;    /* bit-matrix row words (count_kernel 2) */
 } kt_stats;
 
 /* Library and device */
@@ -79,6 +81,7 @@ int kt_malloc(kt_context* ctx, uint64_t bytes, void** dptr);
 int kt_free(kt_context* ctx, void* dptr);
 int kt_memcpy_h2d(kt_context* ctx, void* dst, const void* src, uint64_t bytes);
 int kt_memcpy_d2h(kt_context* ctx, void* dst, const void* src, uint64_t bytes);
+int kt_memcpy_d2d(kt_context* ctx, void* dst, const void* src, uint64_t bytes);
 int kt_synchronize(kt_context* ctx);
 
 /* Whole solves. Pairs are interleaved (u0, v0, u1, v1, ...), u != v, each

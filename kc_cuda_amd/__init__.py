@@ -29,7 +29,7 @@ __all__ = ["Context", "KtResult", "KtError", "kmax_truss", "native_library", "pa
 # Every symbol declared in include/ktruss.h and include/kc_gpu_c.h.
 EXPORTS = [
     "kt_last_error", "kt_device_count", "kt_create", "kt_destroy", "kt_malloc", "kt_free",
-    "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_synchronize", "kt_solve_pairs_device",
+    "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_memcpy_d2d", "kt_synchronize", "kt_solve_pairs_device",
     "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges",
     "kt_parse_tsv", "kt_set_tuning", "kt_last_csr", "kt_last_stats",
 ]
@@ -58,7 +58,8 @@ class KtResult(ctypes.Structure):
 class KtStats(ctypes.Structure):
     _fields_ = [("n_core", ctypes.c_uint32), ("max_out_degree", ctypes.c_uint32),
                 ("m_core", ctypes.c_uint64), ("w_core", ctypes.c_uint64),
-                ("w_peeled", ctypes.c_uint64), ("truss_edges", ctypes.c_uint64)]
+                ("w_peeled", ctypes.c_uint64), ("truss_edges", ctypes.c_uint64),
+                ("count_kernel", ctypes.c_uint32), ("core_words", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -83,6 +84,7 @@ def native_library() -> ctypes.CDLL:
         "kt_free": [_V, _V],
         "kt_memcpy_h2d": [_V, _V, _V, _U64],
         "kt_memcpy_d2h": [_V, _V, _V, _U64],
+        "kt_memcpy_d2d": [_V, _V, _V, _U64],
         "kt_synchronize": [_V],
         "kt_solve_pairs_device": [_V, _V, _U64, _U32, ctypes.POINTER(KtResult)],
         "kt_solve_pairs_host": [_V, _V, _U64, _U32, ctypes.POINTER(KtResult)],
@@ -242,6 +244,14 @@ class Context:
 
     def synchronize(self) -> None:
         _check(self._lib.kt_synchronize(self._h))
+
+    def copy_device(self, dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
+        _check(self._lib.kt_memcpy_d2d(self._h, dst.ptr, src.ptr, nbytes))
+
+    def alloc(self, nbytes: int) -> DeviceArray:
+        p = _V()
+        _check(self._lib.kt_malloc(self._h, max(nbytes, 1), ctypes.byref(p)))
+        return DeviceArray(self, p.value, nbytes)
 
 
 def kmax_truss(pairs: np.ndarray, n: int, device: int = 0) -> tuple[int, int]:

@@ -101,6 +101,13 @@ int kt_memcpy_d2h(kt_context* ctx, void* dst, const void* src, uint64_t bytes) {
   });
 }
 
+int kt_memcpy_d2d(kt_context* ctx, void* dst, const void* src, uint64_t bytes) {
+  return guarded([&] {
+    KT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->solver->stream()));
+    KT_HIP(hipStreamSynchronize(ctx->solver->stream()));
+  });
+}
+
 int kt_synchronize(kt_context* ctx) {
   return guarded([&] { KT_HIP(hipStreamSynchronize(ctx->solver->stream())); });
 }

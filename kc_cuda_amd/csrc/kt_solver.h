@@ -186,6 +186,7 @@ class Solver {
   double dense_tri_cost_ = 8.0, sparse_drop_ratio_ = 3.0;
   bool trace_ = false;  // KT_TRACE=1: per-round frontier sizes on stderr
   bool gallop_ = true;
+  uint32_t count_kind_ = 0, count_words_ = 0;  // the last timed support count
   uint32_t walk_levels_ = 16;
   void make_dense(CoreView* cv);
   HeavyList heavy() {

@@ -307,16 +307,21 @@ void Solver::count_support(CoreView* cv, float* ms) {
   const double avg = cv->nc ? double(cv->mc) / cv->nc : 0.0;
   const int lanes = lanes_count_ ? lanes_count_ : auto_lanes(avg);
   KT_HIP(hipEventRecord(ev_[8], stream_));
-  if (cv->dense)
+  uint32_t kind = 1;
+  if (cv->dense) {
+    kind = 2;
     dense_support(cv->src, cv->adj_d, cv->mc, cv->bits, cv->ws, nullptr, cv->sup, stream_);
-  else if (!force_edge_count_ && support_bitmap_fits(cv->nc, cv->max_dout))
+  } else if (!force_edge_count_ && support_bitmap_fits(cv->nc, cv->max_dout)) {
+    kind = 0;
     support_bitmap(cv->off_d, cv->adj_d, cv->nc, cv->max_dout, lanes, cv->sup, ctl_, stream_);
-  else
+  } else
     support_edges(cv->off_d, cv->adj_d, cv->src, cv->nc, cv->mc, lanes, cv->sup, stream_);
   KT_HIP(hipEventRecord(ev_[9], stream_));
-  if (ms) {
+  if (ms) {  // the timed count of the pass-2 core
     KT_HIP(hipEventSynchronize(ev_[9]));
     KT_HIP(hipEventElapsedTime(ms, ev_[8], ev_[9]));
+    count_kind_ = kind;
+    count_words_ = cv->dense ? cv->ws : 0;
   }
 }
 
@@ -687,6 +692,8 @@ void Solver::last_stats(kt_stats* out) {
   std::memset(out, 0, sizeof(*out));
   const CoreView& cv = last_core_;
   out->n_core = cv.nc0;
+  out->count_kernel = count_kind_;
+  out->core_words = count_words_;
   out->m_core = cv.mc0;
   out->truss_edges = last_truss_;
   if (cv.mc0 == 0) return;
