@@ -121,6 +121,27 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
 #define KT_FORCE_PROBES 32
 int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
 
+/* Batched solve of `count` independent graphs (no counterpart in the
+ * reference, which solves one graph per process, src/kc_gpu.cpp:209-347):
+ * their canonical pairs (u < v < n, strictly sorted within each graph) lie
+ * concatenated in HBM at d_pairs; h_offsets (count + 1 pair offsets) and h_ns
+ * (vertex counts) are host arrays. Per graph: d_kmax, d_count (edges in its
+ * kmax-truss) and d_status (0 ok, 1 invalid input); d_keep (nullable) gets 1
+ * for every pair in its graph's kmax-truss. All graphs run in one persistent
+ * launch; KT_BATCH_GRAPH replays a captured hipGraph of it while the
+ * arguments stay the same. Results equal kt_solve_pairs_* on each graph. */
+#define KT_BATCH_GRAPH 1
+int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
+                   const uint32_t* h_ns, uint32_t count, uint32_t* d_kmax, uint64_t* d_count,
+                   uint32_t* d_status, uint8_t* d_keep, int flags);
+/* Autotuner for kt_batch_solve: times up to per_bucket graphs of every size
+ * bucket (floor(log2 edges)) with each lanes-per-edge choice (4..64) and keeps
+ * the fastest per bucket on this context; table41 (nullable) receives the
+ * 41-entry table. The sample's results land in the given arrays. */
+int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
+                      const uint32_t* h_ns, uint32_t count, uint32_t per_bucket, uint32_t* d_kmax,
+                      uint64_t* d_count, uint32_t* d_status, uint8_t* table41);
+
 /* Byte-model terms of the last solve (one extra device pass, untimed). */
 int kt_last_stats(kt_context* ctx, kt_stats* out);
 

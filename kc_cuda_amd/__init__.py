@@ -31,7 +31,8 @@ EXPORTS = [
     "kt_last_error", "kt_device_count", "kt_create", "kt_destroy", "kt_malloc", "kt_free",
     "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_memcpy_d2d", "kt_synchronize", "kt_solve_pairs_device",
     "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges",
-    "kt_parse_tsv", "kt_set_tuning", "kt_last_csr", "kt_last_stats",
+    "kt_parse_tsv", "kt_set_tuning", "kt_last_csr", "kt_last_stats", "kt_batch_solve",
+    "kt_batch_autotune",
 ]
 
 
@@ -97,6 +98,8 @@ def native_library() -> ctypes.CDLL:
         "kt_set_tuning": [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "kt_last_csr": [_V, _V, _V],
         "kt_last_stats": [_V, ctypes.POINTER(KtStats)],
+        "kt_batch_solve": [_V, _V, _V, _V, _U32, _V, _V, _V, _V, ctypes.c_int],
+        "kt_batch_autotune": [_V, _V, _V, _V, _U32, _U32, _V, _V, _V, _V],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -248,10 +251,82 @@ class Context:
     def copy_device(self, dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
         _check(self._lib.kt_memcpy_d2d(self._h, dst.ptr, src.ptr, nbytes))
 
+    def batch_solve(self, dpairs: DeviceArray, offsets: np.ndarray, ns: np.ndarray, keep: bool = False,
+                    graph: bool = False) -> "BatchResult":
+        """Solve every graph of a batch (pairs already in HBM, canonical and
+        sorted within each graph; offsets: count + 1 pair offsets; ns: vertex
+        counts) in one persistent launch. graph: replay a captured hipGraph."""
+        out = _BatchBuffers(self, offsets, ns, keep)
+        try:
+            _check(self._lib.kt_batch_solve(self._h, dpairs.ptr, out.offsets.ctypes.data,
+                                            out.ns.ctypes.data, out.count, out.kmax.ptr,
+                                            out.edges.ptr, out.status.ptr,
+                                            out.keep.ptr if out.keep else None, 1 if graph else 0))
+            return out.fetch()
+        finally:
+            out.free()
+
+    def batch_autotune(self, dpairs: DeviceArray, offsets: np.ndarray, ns: np.ndarray,
+                       per_bucket: int = 16) -> np.ndarray:
+        """Tune lanes per edge for each size bucket on a sample of the batch;
+        returns the 41-entry table (0 = untuned bucket)."""
+        out = _BatchBuffers(self, offsets, ns, False)
+        table = np.zeros(41, dtype=np.uint8)
+        try:
+            _check(self._lib.kt_batch_autotune(self._h, dpairs.ptr, out.offsets.ctypes.data,
+                                               out.ns.ctypes.data, out.count, per_bucket,
+                                               out.kmax.ptr, out.edges.ptr, out.status.ptr,
+                                               table.ctypes.data))
+        finally:
+            out.free()
+        return table
+
     def alloc(self, nbytes: int) -> DeviceArray:
         p = _V()
         _check(self._lib.kt_malloc(self._h, max(nbytes, 1), ctypes.byref(p)))
         return DeviceArray(self, p.value, nbytes)
+
+
+@dataclass
+class BatchResult:
+    kmax: np.ndarray    # per graph
+    edges: np.ndarray   # per graph: edges in its kmax-truss
+    status: np.ndarray  # per graph: 0 ok, 1 invalid input
+    keep: np.ndarray | None  # per pair: 1 when in its graph's kmax-truss
+
+
+class _BatchBuffers:
+    def __init__(self, ctx: "Context", offsets, ns, keep: bool):
+        self.ctx = ctx
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self.ns = np.ascontiguousarray(ns, dtype=np.uint32)
+        self.count = int(self.ns.shape[0])
+        if self.offsets.shape[0] != self.count + 1:
+            raise ValueError("offsets must have count + 1 entries")
+        self.kmax = ctx.alloc(4 * self.count)
+        self.edges = ctx.alloc(8 * self.count)
+        self.status = ctx.alloc(4 * self.count)
+        self.keep = ctx.alloc(int(self.offsets[-1])) if keep else None
+
+    def fetch(self) -> BatchResult:
+        lib = self.ctx._lib
+        k = np.zeros(self.count, dtype=np.uint32)
+        e = np.zeros(self.count, dtype=np.uint64)
+        s = np.zeros(self.count, dtype=np.uint32)
+        for arr, d in ((k, self.kmax), (e, self.edges), (s, self.status)):
+            if arr.nbytes:
+                _check(lib.kt_memcpy_d2h(self.ctx._h, arr.ctypes.data, d.ptr, arr.nbytes))
+        kp = None
+        if self.keep is not None:
+            kp = np.zeros(int(self.offsets[-1]), dtype=np.uint8)
+            if kp.nbytes:
+                _check(lib.kt_memcpy_d2h(self.ctx._h, kp.ctypes.data, self.keep.ptr, kp.nbytes))
+        return BatchResult(k, e, s, kp)
+
+    def free(self) -> None:
+        for d in (self.kmax, self.edges, self.status, self.keep):
+            if d is not None:
+                d.free()
 
 
 def kmax_truss(pairs: np.ndarray, n: int, device: int = 0) -> tuple[int, int]:

@@ -9,11 +9,17 @@
 #include <string>
 
 #include "../../include/ktruss.h"
+#include "kt_batch_host.h"
 #include "kt_solver.h"
 #include "kt_tsv.h"
 
 struct kt_context {
   std::unique_ptr<kt::Solver> solver;
+  std::unique_ptr<kt::BatchRunner> batch;
+  kt::BatchRunner& batcher() {
+    if (!batch) batch.reset(new kt::BatchRunner(solver.get()));
+    return *batch;
+  }
 };
 
 namespace {
@@ -211,6 +217,29 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
   ctx->solver->set_gallop((flags & KT_FORCE_LEVEL_WALK) == 0, (flags & KT_FORCE_PROBES) ? 1u : 16u);
   g_err.clear();
   return 0;
+}
+
+int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets, const uint32_t* h_ns,
+                   uint32_t count, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status, uint8_t* d_keep,
+                   int flags) {
+  if (!ctx || (count && (!d_pairs || !h_offsets || !h_ns || !d_kmax || !d_count || !d_status)))
+    return set_err("kt_batch_solve: NULL argument");
+  return guarded([&] {
+    ctx->batcher().solve(d_pairs, h_offsets, h_ns, count, d_kmax, d_count, d_status, d_keep,
+                         (flags & KT_BATCH_GRAPH) != 0);
+  });
+}
+
+int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets, const uint32_t* h_ns,
+                      uint32_t count, uint32_t per_bucket, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status,
+                      uint8_t* table41) {
+  if (!ctx || (count && (!d_pairs || !h_offsets || !h_ns || !d_kmax || !d_count || !d_status)))
+    return set_err("kt_batch_autotune: NULL argument");
+  return guarded([&] {
+    ctx->batcher().autotune(d_pairs, h_offsets, h_ns, count, per_bucket ? per_bucket : 16, d_kmax, d_count,
+                            d_status);
+    if (table41) ctx->batcher().table(table41);
+  });
 }
 
 int kt_last_stats(kt_context* ctx, kt_stats* out) {

@@ -136,11 +136,114 @@ std::vector<uint32_t> permutation(uint32_t n, uint64_t key) {
   return p;
 }
 
+// One small graph of a batch, generated serially (the batch runs graphs in
+// parallel): ER G(n, avg/(n-1)) or R-MAT on 2^ceil(log2 n) ids with ids >= n
+// rejected and a random relabel, plus a planted clique; sorted canonical keys.
+std::vector<uint64_t> batch_graph(uint32_t n, bool rmat, double avg_deg, uint32_t clique, uint64_t key) {
+  std::vector<uint64_t> keys;
+  if (n >= 2 && !rmat) {
+    const double p = std::min(1.0, avg_deg / double(n - 1));
+    const double lq = std::log1p(-std::min(p, 1.0 - 1e-16));
+    uint64_t ctr = 0;
+    for (int64_t u = 0; u + 1 < int64_t(n); ++u) {
+      int64_t v = u;
+      while (p > 0.0) {
+        const double r = u01(rnd(key, ctr++));
+        v += 1 + ((p >= 1.0) ? 0 : int64_t(std::floor(std::log1p(-r) / lq)));
+        if (v >= int64_t(n)) break;
+        keys.push_back(canon(uint32_t(u), uint32_t(v)));
+      }
+    }
+  } else if (n >= 2) {
+    int scale = 1;
+    while ((uint64_t(1) << scale) < n) ++scale;
+    const std::vector<uint32_t> perm = permutation(n, fmix64(key ^ 0x5bd1e995ull));
+    const uint64_t samples = uint64_t(double(n) * avg_deg / 2.0);
+    const uint64_t ta = uint64_t(0.57 * 4294967296.0), tb = uint64_t(0.76 * 4294967296.0),
+                   tc = uint64_t(0.95 * 4294967296.0);
+    for (uint64_t i = 0; i < samples; ++i) {
+      uint32_t u = 0, v = 0;
+      uint64_t r = 0;
+      for (int lvl = 0; lvl < scale; ++lvl) {
+        if ((lvl & 1) == 0) r = rnd(key, i * 32 + uint64_t(lvl >> 1));
+        const uint64_t x = (lvl & 1) ? (r >> 32) : (r & 0xffffffffULL);
+        const uint32_t bit = uint32_t(1) << (scale - 1 - lvl);
+        if (x < ta) {
+        } else if (x < tb) {
+          v |= bit;
+        } else if (x < tc) {
+          u |= bit;
+        } else {
+          u |= bit;
+          v |= bit;
+        }
+      }
+      if (u >= n || v >= n || u == v) continue;
+      keys.push_back(canon(perm[u], perm[v]));
+    }
+  }
+  const std::vector<uint32_t> cl = pick_distinct(n, std::min(clique, n), fmix64(key ^ 0x9e3779b9ull));
+  add_clique(keys, cl);
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  return keys;
+}
+
 }  // namespace
 
 extern "C" {
 
 void ktgen_free(void* p) { std::free(p); }
+
+// A batch of `count` independent graphs: n log-uniform in [n_min, n_max], ER or
+// R-MAT (a, b, c) = (.57, .19, .19) with equal odds, average degree uniform in
+// [deg_min, deg_max], planted clique uniform in [clique_min, clique_max]
+// (capped at n). Outputs, malloc'd: pairs (canonical, sorted within each
+// graph, graphs concatenated), offsets (count + 1 pair offsets) and n per
+// graph. Returns the total number of pairs, or -1.
+int64_t ktgen_batch(uint32_t count, uint32_t n_min, uint32_t n_max, double deg_min, double deg_max,
+                    uint32_t clique_min, uint32_t clique_max, uint64_t seed, uint32_t** pairs_out,
+                    uint64_t** off_out, uint32_t** n_out) {
+  std::vector<std::vector<uint64_t>> gs(count);
+  std::vector<uint32_t> ns(count);
+  const double l0 = std::log(double(n_min)), l1 = std::log(double(n_max));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t i = 0; i < int64_t(count); ++i) {
+    const uint64_t key = stream_key(seed, 100 + uint64_t(i));
+    const double a = u01(rnd(key, 0)), b = u01(rnd(key, 1)), c = u01(rnd(key, 2)), d = u01(rnd(key, 3));
+    uint32_t n = uint32_t(std::llround(std::exp(l0 + (l1 - l0) * a)));
+    n = std::min(std::max(n, n_min), n_max);
+    const bool rmat = b < 0.5;
+    const double deg = deg_min + (deg_max - deg_min) * c;
+    const uint32_t cl = clique_min + uint32_t(d * double(clique_max - clique_min + 1));
+    ns[size_t(i)] = n;
+    gs[size_t(i)] = batch_graph(n, rmat, deg, std::min(cl, clique_max), fmix64(key ^ 0xabcdefull));
+  }
+  uint64_t total = 0;
+  uint64_t* off = static_cast<uint64_t*>(std::malloc(sizeof(uint64_t) * (size_t(count) + 1)));
+  uint32_t* nn = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * (count ? count : 1)));
+  if (!off || !nn) return -1;
+  for (uint32_t i = 0; i < count; ++i) {
+    off[i] = total;
+    nn[i] = ns[i];
+    total += gs[i].size();
+  }
+  off[count] = total;
+  uint32_t* p = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * 2 * (total ? total : 1)));
+  if (!p) return -1;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < int64_t(count); ++i) {
+    const auto& k = gs[size_t(i)];
+    for (size_t j = 0; j < k.size(); ++j) {
+      p[2 * (off[i] + j)] = uint32_t(k[j] >> 32);
+      p[2 * (off[i] + j) + 1] = uint32_t(k[j]);
+    }
+  }
+  *pairs_out = p;
+  *off_out = off;
+  *n_out = nn;
+  return int64_t(total);
+}
 
 // Erdos-Renyi G(n, p) with p = avg_deg / (n - 1), by geometric skipping per
 // row, plus a planted clique on `clique` distinct random vertices.

@@ -12,6 +12,8 @@ The named workloads mirror the benchmark configurations:
 * ``er_clique``   ER G(16384, avg deg 16) + 24-clique, seed 0
 * ``rmat``        RMAT scale 22, ef 16, (.57,.19,.19,.05), relabelled, seed 1
 * ``chung_lu``    power-law gamma 2.1, n 50M, ~500M edges + 60-clique, seed 2
+* ``batch``       4096 graphs, n log-uniform in [64, 65536], ER or R-MAT,
+                  average degree U[8, 32], clique U[4, 40], seed 3
 """
 from __future__ import annotations
 
@@ -38,6 +40,10 @@ def _gen():
         lib.ktgen_chung_lu.restype = ctypes.c_int64
         lib.ktgen_chung_lu.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_int64, ctypes.c_int,
                                        ctypes.c_uint32, ctypes.c_uint64, _u32pp]
+        lib.ktgen_batch.restype = ctypes.c_int64
+        lib.ktgen_batch.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                    _u32pp, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint64)), _u32pp]
         lib.ktgen_write_tsv.restype = ctypes.c_int64
         lib.ktgen_write_tsv.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]
         lib.ktgen_free.argtypes = [ctypes.c_void_p]
@@ -79,6 +85,30 @@ def chung_lu(n: int = 50_000_000, gamma: float = 2.1, samples: int = 505_000_000
     p = ctypes.POINTER(ctypes.c_uint32)()
     m = lib.ktgen_chung_lu(n, gamma, samples, int(relabel), clique, seed, ctypes.byref(p))
     return _take(lib, m, p), n
+
+
+def batch(count: int = 4096, n_min: int = 64, n_max: int = 65536, deg_min: float = 8.0,
+          deg_max: float = 32.0, clique_min: int = 4, clique_max: int = 40, seed: int = 3):
+    """A batch of independent graphs (ER or R-MAT, planted cliques).
+
+    Returns (pairs, offsets, ns): pairs (M, 2) uint32 canonical and sorted
+    within each graph, graphs concatenated; offsets (count + 1) uint64 pair
+    offsets; ns (count,) uint32 vertex counts."""
+    lib = _gen()
+    p = ctypes.POINTER(ctypes.c_uint32)()
+    o = ctypes.POINTER(ctypes.c_uint64)()
+    q = ctypes.POINTER(ctypes.c_uint32)()
+    m = lib.ktgen_batch(count, n_min, n_max, deg_min, deg_max, clique_min, clique_max, seed,
+                        ctypes.byref(p), ctypes.byref(o), ctypes.byref(q))
+    if m < 0:
+        raise MemoryError("graph generator ran out of memory")
+    try:
+        offs = np.ctypeslib.as_array(o, shape=(count + 1,)).copy()
+        ns = np.ctypeslib.as_array(q, shape=(count,)).copy() if count else np.zeros(0, np.uint32)
+    finally:
+        lib.ktgen_free(o)
+        lib.ktgen_free(q)
+    return _take(lib, m, p), offs, ns
 
 
 def write_tsv(path: str | os.PathLike, pairs: np.ndarray) -> int:

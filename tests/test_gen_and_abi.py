@@ -28,6 +28,25 @@ def test_generators_deterministic_and_simple():
         assert np.all(np.diff(keys.astype(np.int64) if keys.max() < 2**63 else keys) > 0)
 
 
+def test_batch_generator_layout_and_cliques():
+    """Batches: per-graph canonical sorted pairs, n in range, deterministic;
+    every graph's kmax (oracle) is at least its planted clique's size floor."""
+    from kc_cuda_amd import gen
+    p, offs, ns = gen.batch(40, 16, 2000, 4.0, 20.0, 5, 12, 9)
+    p2, offs2, ns2 = gen.batch(40, 16, 2000, 4.0, 20.0, 5, 12, 9)
+    np.testing.assert_array_equal(p, p2)
+    np.testing.assert_array_equal(offs, offs2)
+    assert offs.shape == (41,) and offs[0] == 0 and offs[-1] == len(p)
+    assert np.all((ns >= 16) & (ns <= 2000))
+    for g in range(40):
+        q = p[offs[g]:offs[g + 1]]
+        assert np.all(q[:, 0] < q[:, 1]) and np.all(q[:, 1] < ns[g])
+        keys = q[:, 0].astype(np.int64) << 32 | q[:, 1].astype(np.int64)
+        assert np.all(np.diff(keys) > 0)
+        if g < 8:
+            assert O.solve_pairs(q, int(ns[g])).kmax >= min(5, int(ns[g]))
+
+
 def test_er_density_and_clique():
     from kc_cuda_amd import gen
     p, n = gen.er_clique(16384, 16.0, 24, 0)
