@@ -2,12 +2,28 @@
 """Benchmark of the k-max-truss hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat22]
+    python bench.py --mode batch [--steps K] [--warmup W] [--slots S]
+    python bench.py --mode serve [--queries Q] [--slots S] [--load F]
 
 A step is one full solve — CSR build, vertex peel, both passes of support
 counting and edge peeling — from an undirected edge list already resident in
 HBM to (kmax, |T|) on the host. The default workload is RMAT scale 22, edge
 factor 16, (a,b,c,d) = (.57,.19,.19,.05), symmetrised, deduplicated,
 self-loops dropped, randomly relabelled, seed 1 (64,152,088 edges).
+
+--mode batch: 4096 graphs (n log-uniform in [64, 65536], ER or R-MAT,
+average degree U[8, 32], planted clique U[4, 40], seed 3), resident in HBM,
+solved through the serving loop with every graph submitted at once; a step is
+one pass over all of them and value = graphs/s. Beside it, sequential plain
+solves of a size-stratified sample of the same graphs (every 32nd by size,
+device-resident) give the sequential graphs/s, and their answers must equal
+the served ones.
+
+--mode serve: serving queries from host memory (lognormal edge counts, median
+200k, sigma 1.5, cap 50M; R-MAT or Chung-Lu with a planted 8..64-clique, seed
+4): closed-loop capacity with one slot and with --slots slots (answers must
+agree query by query), then Poisson arrivals at --load times that capacity;
+value = graphs/s of the Poisson run, with p50/p99 latency (submit to answer).
 
 Prints one JSON line on stdout (everything else goes to stderr):
   value        edges/s = m / wall time per solve
@@ -96,6 +112,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat22")
+    ap.add_argument("--mode", choices=("plain", "batch", "serve"), default="plain")
+    ap.add_argument("--slots", type=int, default=8, help="serving slots (batch and serve modes)")
+    ap.add_argument("--budget-gib", type=int, default=96, help="serving HBM budget")
+    ap.add_argument("--queries", type=int, default=160, help="serve mode: queries in the mix")
+    ap.add_argument("--load", type=float, default=0.7,
+                    help="serve mode: Poisson rate as a fraction of the closed-loop capacity")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=120.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -107,6 +129,11 @@ def main() -> int:
         print("bench.py measures one GPU: the k-max-truss path here runs on a single MI355X "
               "(multi-GPU sharding is a design note only); rerun with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.mode == "batch":
+        return batch_main(args)
+    if args.mode == "serve":
+        return serve_main(args)
 
     from kc_cuda_amd import Context, gen
 
@@ -192,6 +219,168 @@ def main() -> int:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": count_bytes},
         "cpu_baseline": base,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def _pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else 0.0
+
+
+def batch_main(args) -> int:
+    """The 4096-graph mix, device-resident, solved through the serving loop
+    (closed loop: every graph submitted at once) against sequential plain
+    solves of a size-stratified sample."""
+    import numpy as np
+    from kc_cuda_amd import Context, DeviceArray, Server, gen
+
+    t0 = time.perf_counter()
+    pairs, offs, ns = gen.batch()
+    count = int(len(ns))
+    total = int(offs[-1])
+    log(f"[bench] batch: {count} graphs, {total} edges, generated in {time.perf_counter() - t0:.1f}s")
+    ctx = Context(0)
+    d = ctx.to_device(pairs)
+    views = [DeviceArray(ctx, d.ptr + 8 * int(offs[g]), 8 * int(offs[g + 1] - offs[g])) for g in range(count)]
+    srv = Server(0, args.slots, args.budget_gib << 30)
+
+    def one_pass():
+        tickets = [srv.submit(views[g], int(ns[g])) for g in range(count)]
+        return [srv.wait(t) for t in tickets]
+
+    for i in range(args.warmup):
+        t1 = time.perf_counter()
+        one_pass()
+        log(f"[bench] warmup {i}: {time.perf_counter() - t1:.2f}s")
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        res = one_pass()
+    wall = (time.perf_counter() - t1) / max(args.steps, 1)
+    bad = [r for r in res if r.status != 0]
+    if bad:
+        log(f"[bench] {len(bad)} graphs failed: {bad[0].error}")
+        return 3
+    counters = srv.counters()
+    log(f"[bench] served: {wall * 1e3:.1f} ms per pass, {count / wall:.0f} graphs/s, "
+        f"{args.slots} slots, counters {counters}")
+    # sequential plain solves of a size-stratified sample (every 32nd by size)
+    by_size = np.argsort(-(offs[1:] - offs[:-1]).astype(np.int64), kind="stable")
+    sample = by_size[::32]
+    t_seq = 0.0
+    for g in sample:
+        t2 = time.perf_counter()
+        r = ctx.solve_device(views[g], views[g].nbytes // 8, int(ns[g]))
+        t_seq += time.perf_counter() - t2
+        if (r.kmax, r.truss_edges) != (res[g].kmax, res[g].truss_edges):
+            log(f"[bench] MISMATCH graph {g}: plain {r.kmax},{r.truss_edges} served "
+                f"{res[g].kmax},{res[g].truss_edges}")
+            return 3
+    seq_gps = len(sample) / t_seq
+    log(f"[bench] sequential plain: {len(sample)} sampled graphs in {t_seq:.2f}s -> {seq_gps:.1f} graphs/s")
+    srv.close()
+    d.free()
+    ctx.close()
+    line = {
+        "metric": METRIC, "value": count / wall, "unit": "graphs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "batch4096", "graphs": count, "edges": total,
+                   "generator": "n log-uniform [64, 65536], ER or R-MAT(.57,.19,.19,.05) 50/50, "
+                                "avg degree U[8,32], clique U[4,40], seed 3",
+                   "engine": f"serving loop, {args.slots} slots, closed loop, inputs in HBM",
+                   "edges_per_s": total / wall,
+                   "sequential_plain_graphs_per_s": seq_gps,
+                   "sequential_sample": f"{len(sample)} graphs, every 32nd by size, device-resident",
+                   "speedup_vs_sequential": (count / wall) / seq_gps,
+                   "peak_in_flight": counters["peak_in_flight"]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def serve_main(args) -> int:
+    """Serving queries (lognormal sizes, median 200k edges, sigma 1.5, cap
+    50M; R-MAT or Chung-Lu with planted cliques of 8-64, seed 4) from host
+    memory: closed-loop capacity with 1 slot and with --slots slots (their
+    answers must agree), then Poisson arrivals at --load times that capacity
+    for p50/p99 latency."""
+    import threading
+    import numpy as np
+    from kc_cuda_amd import Server, gen
+
+    t0 = time.perf_counter()
+    mix = gen.serve_mix(args.queries)
+    edges = sum(len(p) for p, _ in mix)
+    sizes = np.array([len(p) for p, _ in mix])
+    log(f"[bench] serve: {len(mix)} queries, {edges} edges (median {int(np.median(sizes))}, "
+        f"max {sizes.max()}), generated in {time.perf_counter() - t0:.1f}s")
+
+    def closed_loop(srv):
+        t1 = time.perf_counter()
+        tickets = [srv.submit(p, n) for p, n in mix]
+        res = [srv.wait(t) for t in tickets]
+        return time.perf_counter() - t1, res
+
+    budget = args.budget_gib << 30
+    with Server(0, 1, budget) as one:
+        closed_loop(one)  # warm: buffers grow to the largest query
+        t_one, res_one = closed_loop(one)
+    log(f"[bench] 1 slot: {len(mix) / t_one:.1f} queries/s")
+    with Server(0, args.slots, budget) as srv:
+        closed_loop(srv)
+        t_many, res_many = closed_loop(srv)
+        cap = len(mix) / t_many
+        log(f"[bench] {args.slots} slots: {cap:.1f} queries/s closed loop")
+        for i, (a, b) in enumerate(zip(res_one, res_many)):
+            if a.status or b.status or (a.kmax, a.truss_edges) != (b.kmax, b.truss_edges):
+                log(f"[bench] MISMATCH query {i}: alone {a.kmax},{a.truss_edges} in the mix "
+                    f"{b.kmax},{b.truss_edges} ({a.error}{b.error})")
+                return 3
+        rate = args.load * cap
+        rng = np.random.default_rng(4)
+        gaps = rng.exponential(1.0 / rate, len(mix))
+        tickets = [0] * len(mix)
+
+        def feed():
+            start = time.perf_counter()
+            at = 0.0
+            for i, (p, n) in enumerate(mix):
+                at += gaps[i]
+                delay = start + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[i] = srv.submit(p, n)
+
+        t1 = time.perf_counter()
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        res = [srv.wait(t) for t in tickets]
+        wall = time.perf_counter() - t1
+        counters = srv.counters()
+    lat = [r.total_ms for r in res]
+    run = [r.run_ms for r in res]
+    line = {
+        "metric": METRIC, "value": len(mix) / wall, "unit": "graphs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "serve", "queries": len(mix), "edges": int(edges),
+                   "generator": "lognormal edges (median 200k, sigma 1.5, cap 50M), R-MAT or "
+                                "Chung-Lu 50/50, clique U[8,64], seed 4; Poisson arrivals",
+                   "slots": args.slots, "hbm_budget_gib": args.budget_gib,
+                   "arrival_rate_per_s": rate, "load": args.load,
+                   "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+                   "p50_run_ms": _pct(run, 50), "p99_run_ms": _pct(run, 99),
+                   "capacity_graphs_per_s": cap, "one_slot_graphs_per_s": len(mix) / t_one,
+                   "speedup_vs_one_slot": t_one / t_many,
+                   "peak_in_flight": counters["peak_in_flight"],
+                   "peak_gib_charged": counters["peak_bytes_in_use"] / 2**30,
+                   "over_estimate": counters["over_estimate"]},
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     return 0

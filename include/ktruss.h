@@ -129,7 +129,10 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
  * kmax-truss) and d_status (0 ok, 1 invalid input); d_keep (nullable) gets 1
  * for every pair in its graph's kmax-truss. All graphs run in one persistent
  * launch; KT_BATCH_GRAPH replays a captured hipGraph of it while the
- * arguments stay the same. Results equal kt_solve_pairs_* on each graph. */
+ * arguments stay the same. Results equal kt_solve_pairs_* on each graph.
+ * Currently switched off: both batch entry points return an error (the
+ * kernel raised an unexplained GPU memory fault on its parity test); the
+ * kt_serve_* entry points solve many graphs concurrently instead. */
 #define KT_BATCH_GRAPH 1
 int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
                    const uint32_t* h_ns, uint32_t count, uint32_t* d_kmax, uint64_t* d_count,
@@ -141,6 +144,50 @@ int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_o
 int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
                       const uint32_t* h_ns, uint32_t count, uint32_t per_bucket, uint32_t* d_kmax,
                       uint64_t* d_count, uint32_t* d_status, uint8_t* table41);
+
+/* Serving (no counterpart in the reference, which answers one graph per
+ * process, src/kc_gpu.cpp:209-347): queries are queued first come, first
+ * served and solved concurrently on `slots` solver contexts, each with its
+ * own HIP stream and host thread. Each slot keeps its CSR, support and peel
+ * buffers resident between queries and is charged against hbm_budget what
+ * they hold, or the footprint of its running query (kt_serve_footprint) if
+ * larger. A query is admitted as soon as a slot is free and that charge fits
+ * in the budget; when nothing runs and it still does not fit, idle slots
+ * free their buffers. A query that could not fit even then is rejected
+ * (status 1). Results equal kt_solve_pairs_* on the same graph, alone or in
+ * the mix. */
+typedef struct kt_server kt_server;
+typedef struct kt_serve_result {
+  int32_t status;         /* 0 solved, 1 rejected by admission, 2 solve failed */
+  uint32_t slot;          /* slot that solved it */
+  kt_result result;
+  uint64_t footprint;     /* kt_serve_footprint of the query */
+  uint64_t charged;       /* bytes its slot was charged while it ran */
+  uint64_t slot_bytes;    /* device bytes its slot held after the solve */
+  double queued_ms;       /* submit -> admitted */
+  double run_ms;          /* admitted -> solved, upload included */
+  double total_ms;        /* submit -> solved */
+} kt_serve_result;
+typedef struct kt_serve_counters {
+  uint64_t submitted, completed, rejected, failed;
+  uint64_t in_flight, peak_in_flight;
+  uint64_t bytes_in_use, peak_bytes_in_use;
+  uint64_t over_estimate; /* solves whose slot grew past the bytes charged for them */
+  uint64_t trims;         /* idle slots that freed their buffers to let a query in */
+} kt_serve_counters;
+int kt_serve_create(int device, int slots, uint64_t hbm_budget, kt_server** out);
+int kt_serve_destroy(kt_server* srv);
+/* pairs: canonical pairs (u < v < n) in host memory, or device memory when
+ * on_device is 1; they must stay valid until kt_serve_wait returns for the
+ * ticket. */
+int kt_serve_submit(kt_server* srv, const uint32_t* pairs, uint64_t m, uint32_t n, int on_device,
+                    uint64_t* ticket);
+/* Blocks until the query is done; its error text (status 2) is then
+ * kt_last_error(). Each ticket can be waited for once. */
+int kt_serve_wait(kt_server* srv, uint64_t ticket, kt_serve_result* out);
+int kt_serve_counters_get(kt_server* srv, kt_serve_counters* out);
+/* Admission charge of a query of m edges on n vertices (bytes). */
+uint64_t kt_serve_footprint(uint64_t m, uint32_t n);
 
 /* Byte-model terms of the last solve (one extra device pass, untimed). */
 int kt_last_stats(kt_context* ctx, kt_stats* out);

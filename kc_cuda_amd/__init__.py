@@ -23,8 +23,8 @@ import numpy as np
 from . import _lib
 from ._lib import NativeLibraryMissing  # noqa: F401
 
-__all__ = ["Context", "KtResult", "KtError", "kmax_truss", "native_library", "parse_tsv",
-           "NativeLibraryMissing", "EXPORTS"]
+__all__ = ["Context", "KtResult", "KtError", "Server", "ServeResult", "kmax_truss", "native_library",
+           "parse_tsv", "NativeLibraryMissing", "EXPORTS"]
 
 # Every symbol declared in include/ktruss.h and include/kc_gpu_c.h.
 EXPORTS = [
@@ -32,7 +32,8 @@ EXPORTS = [
     "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_memcpy_d2d", "kt_synchronize", "kt_solve_pairs_device",
     "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges",
     "kt_parse_tsv", "kt_set_tuning", "kt_last_csr", "kt_last_stats", "kt_batch_solve",
-    "kt_batch_autotune",
+    "kt_batch_autotune", "kt_serve_create", "kt_serve_destroy", "kt_serve_submit", "kt_serve_wait",
+    "kt_serve_counters_get", "kt_serve_footprint",
 ]
 
 
@@ -64,6 +65,22 @@ class KtStats(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class KtServeResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("slot", ctypes.c_uint32), ("result", KtResult),
+                ("footprint", ctypes.c_uint64), ("charged", ctypes.c_uint64),
+                ("slot_bytes", ctypes.c_uint64), ("queued_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
+
+
+class KtServeCounters(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "submitted", "completed", "rejected", "failed", "in_flight", "peak_in_flight",
+        "bytes_in_use", "peak_bytes_in_use", "over_estimate", "trims")]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
 _V = ctypes.c_void_p
@@ -100,11 +117,18 @@ def native_library() -> ctypes.CDLL:
         "kt_last_stats": [_V, ctypes.POINTER(KtStats)],
         "kt_batch_solve": [_V, _V, _V, _V, _U32, _V, _V, _V, _V, ctypes.c_int],
         "kt_batch_autotune": [_V, _V, _V, _V, _U32, _U32, _V, _V, _V, _V],
+        "kt_serve_create": [ctypes.c_int, ctypes.c_int, _U64, ctypes.POINTER(_V)],
+        "kt_serve_destroy": [_V],
+        "kt_serve_submit": [_V, _V, _U64, _U32, ctypes.c_int, ctypes.POINTER(_U64)],
+        "kt_serve_wait": [_V, _U64, ctypes.POINTER(KtServeResult)],
+        "kt_serve_counters_get": [_V, ctypes.POINTER(KtServeCounters)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
+    lib.kt_serve_footprint.argtypes = [_U64, _U32]
+    lib.kt_serve_footprint.restype = _U64
     lib._kt_typed = True
     return lib
 
@@ -256,15 +280,17 @@ class Context:
         """Solve every graph of a batch (pairs already in HBM, canonical and
         sorted within each graph; offsets: count + 1 pair offsets; ns: vertex
         counts) in one persistent launch. graph: replay a captured hipGraph."""
-        out = _BatchBuffers(self, offsets, ns, keep)
+        run = self.batch(dpairs, offsets, ns, keep)
         try:
-            _check(self._lib.kt_batch_solve(self._h, dpairs.ptr, out.offsets.ctypes.data,
-                                            out.ns.ctypes.data, out.count, out.kmax.ptr,
-                                            out.edges.ptr, out.status.ptr,
-                                            out.keep.ptr if out.keep else None, 1 if graph else 0))
-            return out.fetch()
+            run.solve(graph)
+            return run.result()
         finally:
-            out.free()
+            run.free()
+
+    def batch(self, dpairs: DeviceArray, offsets: np.ndarray, ns: np.ndarray,
+              keep: bool = False) -> "BatchRun":
+        """A reusable batch: result buffers stay allocated across solves."""
+        return BatchRun(self, dpairs, offsets, ns, keep)
 
     def batch_autotune(self, dpairs: DeviceArray, offsets: np.ndarray, ns: np.ndarray,
                        per_bucket: int = 16) -> np.ndarray:
@@ -298,6 +324,7 @@ class BatchResult:
 class _BatchBuffers:
     def __init__(self, ctx: "Context", offsets, ns, keep: bool):
         self.ctx = ctx
+        self.dpairs = None
         self.offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         self.ns = np.ascontiguousarray(ns, dtype=np.uint32)
         self.count = int(self.ns.shape[0])
@@ -327,6 +354,115 @@ class _BatchBuffers:
         for d in (self.kmax, self.edges, self.status, self.keep):
             if d is not None:
                 d.free()
+
+
+class BatchRun(_BatchBuffers):
+    """Device result buffers of one batch, reused across solves."""
+
+    def __init__(self, ctx: "Context", dpairs: DeviceArray, offsets, ns, keep: bool):
+        super().__init__(ctx, offsets, ns, keep)
+        self.dpairs = dpairs
+
+    def solve(self, graph: bool = False) -> None:
+        lib = self.ctx._lib
+        _check(lib.kt_batch_solve(self.ctx._h, self.dpairs.ptr, self.offsets.ctypes.data,
+                                  self.ns.ctypes.data, self.count, self.kmax.ptr, self.edges.ptr,
+                                  self.status.ptr, self.keep.ptr if self.keep else None,
+                                  1 if graph else 0))
+
+    def result(self) -> BatchResult:
+        return self.fetch()
+
+
+@dataclass
+class ServeResult:
+    ticket: int
+    status: int          # 0 solved, 1 rejected by admission, 2 solve failed
+    slot: int
+    result: KtResult
+    footprint: int
+    charged: int
+    slot_bytes: int
+    queued_ms: float
+    run_ms: float
+    total_ms: float
+    error: str
+
+    @property
+    def kmax(self) -> int:
+        return int(self.result.kmax)
+
+    @property
+    def truss_edges(self) -> int:
+        return int(self.result.truss_edges)
+
+
+class Server:
+    """Serving loop over the native library: graphs are queued first come,
+    first served and solved concurrently on `slots` solver contexts (one HIP
+    stream and host thread each); a query is admitted once a slot is free and
+    its charge (the larger of its footprint and what the slot holds) fits in
+    `hbm_budget` bytes beside the other slots' charges.
+
+        with Server(slots=8, hbm_budget=64 << 30) as srv:
+            t = srv.submit(pairs, n)
+            r = srv.wait(t)          # ServeResult: kmax, truss_edges, latencies
+    """
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 64 << 30):
+        self._lib = native_library()
+        h = _V()
+        _check(self._lib.kt_serve_create(device, slots, hbm_budget, ctypes.byref(h)))
+        self._h = h
+        self.slots = slots
+        self.hbm_budget = hbm_budget
+        self._held = {}  # ticket -> input kept alive until waited for
+
+    @staticmethod
+    def footprint(m: int, n: int) -> int:
+        return int(native_library().kt_serve_footprint(m, n))
+
+    def submit(self, pairs, n: int) -> int:
+        """pairs: (m, 2) uint32 canonical pairs on the host, or a DeviceArray of
+        them (then pass m via pairs.nbytes // 8)."""
+        t = _U64()
+        if isinstance(pairs, DeviceArray):
+            m = pairs.nbytes // 8
+            _check(self._lib.kt_serve_submit(self._h, pairs.ptr, m, n, 1, ctypes.byref(t)))
+        else:
+            p = _u32(pairs).reshape(-1, 2)
+            m = p.shape[0]
+            _check(self._lib.kt_serve_submit(self._h, p.ctypes.data if m else None, m, n, 0,
+                                             ctypes.byref(t)))
+        self._held[t.value] = pairs if isinstance(pairs, DeviceArray) else p
+        return t.value
+
+    def wait(self, ticket: int) -> ServeResult:
+        out = KtServeResult()
+        rc = self._lib.kt_serve_wait(self._h, ticket, ctypes.byref(out))
+        self._held.pop(ticket, None)
+        _check(rc)
+        msg = self._lib.kt_last_error() if out.status != 0 else None
+        return ServeResult(ticket, int(out.status), int(out.slot), out.result, int(out.footprint),
+                           int(out.charged), int(out.slot_bytes), float(out.queued_ms), float(out.run_ms),
+                           float(out.total_ms), msg.decode() if msg else "")
+
+    def counters(self) -> dict:
+        c = KtServeCounters()
+        _check(self._lib.kt_serve_counters_get(self._h, ctypes.byref(c)))
+        return c.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            _check(self._lib.kt_serve_destroy(self._h))
+            self._h = None
+            self._held.clear()
+
+    def __enter__(self) -> "Server":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
 
 
 def kmax_truss(pairs: np.ndarray, n: int, device: int = 0) -> tuple[int, int]:

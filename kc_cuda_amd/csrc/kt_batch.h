@@ -4,6 +4,8 @@
 
 namespace kt {
 
+constexpr int kBatchPhases = 10;
+
 struct BatchArgs {
   const uint32_t* pairs;      // all graphs' canonical pairs, interleaved
   const uint64_t* poff;       // count + 1 pair offsets
@@ -16,6 +18,10 @@ struct BatchArgs {
   uint32_t* status;           // per graph: 0 ok, 1 invalid input
   uint8_t* keep;              // per pair: 1 when in the kmax-truss (nullable)
   unsigned long long* ticks;  // per graph solve time in wall-clock ticks (nullable)
+  unsigned long long* phases; // per graph kBatchPhases entries (nullable): ticks at the end of
+                              // build, cores, count 1, peel 1, count 2, peel 2; then
+                              // levels << 32 | rounds of pass 1 and of pass 2; the
+                              // absolute start and end ticks
   uint32_t* slab;             // grid * slab_words scratch words
   uint64_t slab_words;
   uint32_t* work;             // work-stealing counter

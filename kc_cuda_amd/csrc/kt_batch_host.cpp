@@ -3,6 +3,8 @@
 #include "kt_batch_host.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -21,13 +23,14 @@ BatchRunner::BatchRunner(Solver* s) : stream_(s->stream()), device_(s->device())
   KT_HIP(hipSetDevice(device_));
   KT_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_));
   std::fill(std::begin(table_), std::end(table_), uint8_t(0));
+  if (const char* v = std::getenv("KT_BATCH_PROFILE")) profile_ = v[0] == '1';
 }
 
 BatchRunner::~BatchRunner() {
   (void)hipSetDevice(device_);
   (void)hipStreamSynchronize(stream_);
   drop_graph();
-  DevBuf* all[] = {&poff_, &ns_, &order_, &lanes_, &slab_, &work_, &ticks_};
+  DevBuf* all[] = {&poff_, &ns_, &order_, &lanes_, &slab_, &work_, &ticks_, &phases_};
   for (auto* b : all) b->release();
 }
 
@@ -106,9 +109,15 @@ void BatchRunner::solve(const uint32_t* d_pairs, const uint64_t* h_off, const ui
   a.keep = d_keep;
   a.ticks = nullptr;
   if (!count) return;
+  if (profile_) {
+    phases_.ensure(sizeof(unsigned long long) * kBatchPhases * count);
+    KT_HIP(hipMemsetAsync(phases_.p, 0, sizeof(unsigned long long) * kBatchPhases * count, stream_));
+    a.phases = phases_.as<unsigned long long>();
+  }
   if (!use_graph) {
     batch_launch(a, grid, stream_);
     KT_HIP(hipStreamSynchronize(stream_));
+    if (profile_) report_phases(h_off, count);
     return;
   }
   if (!exec_ || grid != cached_grid_ || std::memcmp(&a, &cached_, sizeof(a)) != 0) {
@@ -123,6 +132,67 @@ void BatchRunner::solve(const uint32_t* d_pairs, const uint64_t* h_off, const ui
   }
   KT_HIP(hipGraphLaunch(exec_, stream_));
   KT_HIP(hipStreamSynchronize(stream_));
+  if (profile_) report_phases(h_off, count);
+}
+
+// Sums of the per-graph phase times (ms) and mean levels / rounds, by size
+// bucket floor(log2 m).
+void BatchRunner::report_phases(const uint64_t* h_off, uint32_t count) {
+  int khz = 0;
+  KT_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
+  const double ms_per_tick = khz > 0 ? 1.0 / double(khz) : 1e-5;
+  std::vector<unsigned long long> ph(size_t(kBatchPhases) * count);
+  KT_HIP(hipMemcpy(ph.data(), phases_.p, sizeof(unsigned long long) * ph.size(), hipMemcpyDeviceToHost));
+  struct Acc {
+    uint32_t graphs = 0;
+    double ms[6] = {0, 0, 0, 0, 0, 0};
+    double lv[2] = {0, 0}, rd[2] = {0, 0};
+    double max_ms = 0;
+  };
+  std::vector<Acc> acc(41);
+  unsigned long long t_first = ~0ull, t_last_start = 0, t_end = 0, longest = 0;
+  uint32_t g_longest = 0;
+  double busy = 0;
+  for (uint32_t g = 0; g < count; ++g) {
+    const unsigned long long* p = &ph[size_t(g) * kBatchPhases];
+    Acc& a = acc[bucket_of(h_off[g + 1] - h_off[g])];
+    a.graphs += 1;
+    t_first = std::min(t_first, p[8]);
+    t_last_start = std::max(t_last_start, p[8]);
+    t_end = std::max(t_end, p[9]);
+    busy += double(p[9] - p[8]);
+    a.max_ms = std::max(a.max_ms, double(p[9] - p[8]) * ms_per_tick);
+    if (p[9] - p[8] > longest) {
+      longest = p[9] - p[8];
+      g_longest = g;
+    }
+    unsigned long long prev = 0;
+    for (int i = 0; i < 6; ++i) {
+      const unsigned long long t = std::max(p[i], prev);  // a skipped pass leaves 0
+      a.ms[i] += double(t - prev) * ms_per_tick;
+      prev = t;
+    }
+    for (int k = 0; k < 2; ++k) {
+      a.lv[k] += double(p[6 + k] >> 32);
+      a.rd[k] += double(p[6 + k] & 0xffffffffu);
+    }
+  }
+  std::fprintf(stderr,
+               "[batch] span %.1f ms, last start at %.1f ms, busy %.1f workgroup-s (mean concurrency %.0f); "
+               "longest graph %u: %.1f ms, m %llu\n",
+               double(t_end - t_first) * ms_per_tick, double(t_last_start - t_first) * ms_per_tick,
+               busy * ms_per_tick * 1e-3, busy / double(std::max(1ull, t_end - t_first)), g_longest,
+               double(longest) * ms_per_tick, (unsigned long long)(h_off[g_longest + 1] - h_off[g_longest]));
+  std::fprintf(stderr, "[batch] phase ms summed per bucket: build core count1 peel1 count2 peel2 | "
+                       "mean levels/rounds pass1 pass2 | max graph ms\n");
+  for (uint32_t b = 0; b < 41; ++b) {
+    const Acc& a = acc[b];
+    if (!a.graphs) continue;
+    std::fprintf(stderr,
+                 "[batch] 2^%-2u %5u graphs: %9.1f %9.1f %9.1f %9.1f %9.1f %9.1f | %6.1f/%-7.1f %6.1f/%-7.1f | %8.1f\n",
+                 b, a.graphs, a.ms[0], a.ms[1], a.ms[2], a.ms[3], a.ms[4], a.ms[5], a.lv[0] / a.graphs,
+                 a.rd[0] / a.graphs, a.lv[1] / a.graphs, a.rd[1] / a.graphs, a.max_ms);
+  }
 }
 
 // For every size bucket, up to `per_bucket` graphs of the batch are solved

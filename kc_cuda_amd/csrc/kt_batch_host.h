@@ -28,11 +28,13 @@ class BatchRunner {
   BatchArgs plan(const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns, uint32_t count,
                  const std::vector<uint32_t>& order, const std::vector<uint8_t>& lanes, int* grid);
   void drop_graph();
+  void report_phases(const uint64_t* h_off, uint32_t count);
 
   hipStream_t stream_;
   int device_;
   int cus_ = 0;
-  DevBuf poff_, ns_, order_, lanes_, slab_, work_, ticks_;
+  DevBuf poff_, ns_, order_, lanes_, slab_, work_, ticks_, phases_;
+  bool profile_ = false;  // KT_BATCH_PROFILE=1: per-phase times by size bucket on stderr
   uint8_t table_[41];  // lanes per edge by bucket floor(log2 m); 0 = by degree
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;

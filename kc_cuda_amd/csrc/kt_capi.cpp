@@ -10,6 +10,7 @@
 
 #include "../../include/ktruss.h"
 #include "kt_batch_host.h"
+#include "kt_serve.h"
 #include "kt_solver.h"
 #include "kt_tsv.h"
 
@@ -219,9 +220,20 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
   return 0;
 }
 
+// The persistent batched kernel (one workgroup per graph) is switched off:
+// its parity test raised a GPU memory fault that a CPU build of the same
+// kernel source (one workgroup as 256 host threads, address and undefined-
+// behaviour sanitizers) does not reproduce, so the cause is not known. Both
+// entry points fail with this message until it is.
+constexpr bool kBatchKernelEnabled = false;
+constexpr const char* kBatchOff =
+    "the persistent batched kernel is switched off (unexplained GPU memory fault on its parity test); "
+    "solve the graphs one by one, or concurrently with the kt_serve_* entry points";
+
 int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets, const uint32_t* h_ns,
                    uint32_t count, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status, uint8_t* d_keep,
                    int flags) {
+  if (!kBatchKernelEnabled) return set_err(std::string("kt_batch_solve: ") + kBatchOff);
   if (!ctx || (count && (!d_pairs || !h_offsets || !h_ns || !d_kmax || !d_count || !d_status)))
     return set_err("kt_batch_solve: NULL argument");
   return guarded([&] {
@@ -233,6 +245,7 @@ int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_o
 int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets, const uint32_t* h_ns,
                       uint32_t count, uint32_t per_bucket, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status,
                       uint8_t* table41) {
+  if (!kBatchKernelEnabled) return set_err(std::string("kt_batch_autotune: ") + kBatchOff);
   if (!ctx || (count && (!d_pairs || !h_offsets || !h_ns || !d_kmax || !d_count || !d_status)))
     return set_err("kt_batch_autotune: NULL argument");
   return guarded([&] {
@@ -241,6 +254,73 @@ int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* 
     if (table41) ctx->batcher().table(table41);
   });
 }
+
+struct kt_server {
+  std::unique_ptr<kt::Server> srv;
+};
+
+int kt_serve_create(int device, int slots, uint64_t hbm_budget, kt_server** out) {
+  if (!out) return set_err("kt_serve_create: NULL out");
+  *out = nullptr;
+  return guarded([&] {
+    int count = 0;
+    KT_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) throw kt::Error{hipErrorInvalidValue, "kt_serve_create: no such device"};
+    auto s = std::make_unique<kt_server>();
+    s->srv = std::make_unique<kt::Server>(device, slots, hbm_budget);
+    *out = s.release();
+  });
+}
+
+int kt_serve_destroy(kt_server* srv) {
+  if (!srv) return 0;
+  return guarded([&] { delete srv; });
+}
+
+int kt_serve_submit(kt_server* srv, const uint32_t* pairs, uint64_t m, uint32_t n, int on_device,
+                    uint64_t* ticket) {
+  if (!srv || !ticket) return set_err("kt_serve_submit: NULL argument");
+  return guarded([&] { *ticket = srv->srv->submit(pairs, m, n, on_device != 0); });
+}
+
+int kt_serve_wait(kt_server* srv, uint64_t ticket, kt_serve_result* out) {
+  if (!srv || !out) return set_err("kt_serve_wait: NULL argument");
+  std::string query_err;
+  const int rc = guarded([&] {
+    kt::ServeOutcome o = srv->srv->wait(ticket);
+    query_err = o.error;
+    out->status = o.status;
+    out->slot = o.slot;
+    out->result = o.res;
+    out->footprint = o.footprint;
+    out->charged = o.charged;
+    out->slot_bytes = o.slot_bytes;
+    out->queued_ms = o.queued_ms;
+    out->run_ms = o.run_ms;
+    out->total_ms = o.total_ms;
+  });
+  if (rc == 0 && !query_err.empty()) g_err = query_err;
+  return rc;
+}
+
+int kt_serve_counters_get(kt_server* srv, kt_serve_counters* out) {
+  if (!srv || !out) return set_err("kt_serve_counters_get: NULL argument");
+  return guarded([&] {
+    const kt::ServeCounters c = srv->srv->counters();
+    out->submitted = c.submitted;
+    out->completed = c.completed;
+    out->rejected = c.rejected;
+    out->failed = c.failed;
+    out->in_flight = c.in_flight;
+    out->peak_in_flight = c.peak_in_flight;
+    out->bytes_in_use = c.bytes_in_use;
+    out->peak_bytes_in_use = c.peak_bytes_in_use;
+    out->over_estimate = c.over_estimate;
+    out->trims = c.trims;
+  });
+}
+
+uint64_t kt_serve_footprint(uint64_t m, uint32_t n) { return kt::Server::footprint(m, n); }
 
 int kt_last_stats(kt_context* ctx, kt_stats* out) {
   if (!ctx || !out) return set_err("kt_last_stats: NULL argument");

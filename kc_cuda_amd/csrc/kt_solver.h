@@ -78,6 +78,14 @@ class Solver {
  public:
   explicit Solver(int device);
   ~Solver();
+  // Device bytes this context holds now (its buffers keep their size
+  // between solves, so after a solve this is at least that solve's peak).
+  uint64_t device_bytes() const;
+  // Frees every buffer (they grow again on the next solve); the last solve's
+  // device-side results are gone afterwards.
+  void release_buffers();
+  // Largest edge-id matrix a dense core may allocate (bytes).
+  void set_eidm_budget(uint64_t bytes) { eidm_budget_ = bytes; }
 
   // Input already resident on the device: m canonical pairs (interleaved).
   void solve_pairs_device(const uint32_t* d_pairs, uint64_t m, uint32_t n, kt_result* res);
@@ -150,6 +158,8 @@ class Solver {
   uint32_t n() const { return n_; }
 
  private:
+  template <typename F>
+  void each_buffer(F f) const;  // every device buffer the context owns
   void build_from_pairs(const uint32_t* d_pairs, uint64_t m, uint32_t n);
   void build_from_lines(const uint32_t* d_xs, const uint32_t* d_ys, uint64_t L, uint32_t n);
   bool build_partitioned(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint32_t n);

@@ -49,30 +49,37 @@ Solver::Solver(int device) : device_(device) {
   if (const char* v = std::getenv("KT_EIDM_BUDGET_MB")) eidm_budget_ = uint64_t(std::atoll(v)) << 20;
 }
 
+template <typename F>
+void Solver::each_buffer(F f) const {
+  const DevBuf* all[] = {&off_, &adj_, &deg_, &lvl_, &q0_, &q1_, &deg_bak_, &cursor_, &list_, &new_id_,
+                         &keys_, &keys2_, &vals_, &vals2_, &to_, &inv_, &inv_orig_, &dout_, &off_d_,
+                         &adj_d_, &src_, &din_, &din_off_, &tkeys_, &tvals_, &tiota_, &in_src_, &off_u_,
+                         &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &del_thr_, &orig0_,
+                         &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &fbits_, &pref0_, &eidm_, &heavy_,
+                         &vflag_, &vmap_, &roff_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_};
+  for (const DevBuf* b : all) f(*b);
+  for (auto& w : work_)
+    for (const DevBuf* b : {&w.off, &w.adj, &w.src, &w.orig, &w.sup}) f(*b);
+  for (auto& l : levels_)
+    for (const DevBuf* b : {&l->b_off, &l->b_adj, &l->b_deg, &l->b_lvl, &l->b_orig}) f(*b);
+}
+
+uint64_t Solver::device_bytes() const {
+  uint64_t total = sizeof(Ctl);
+  each_buffer([&](const DevBuf& b) { total += b.bytes; });
+  return total;
+}
+
+void Solver::release_buffers() {
+  KT_HIP(hipSetDevice(device_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  each_buffer([](const DevBuf& b) { const_cast<DevBuf&>(b).release(); });
+}
+
 Solver::~Solver() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  DevBuf* all[] = {&off_, &adj_, &deg_, &lvl_, &q0_, &q1_, &deg_bak_, &cursor_, &list_, &new_id_,
-                   &keys_, &keys2_, &vals_, &vals2_, &to_, &inv_, &inv_orig_, &dout_, &off_d_,
-                   &adj_d_, &src_, &din_, &din_off_, &tkeys_, &tvals_, &tiota_, &in_src_, &off_u_,
-                   &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &del_thr_, &orig0_,
-                   &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &fbits_, &pref0_, &eidm_, &heavy_,
-                   &vflag_, &vmap_, &roff_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_};
-  for (auto* b : all) b->release();
-  for (auto& w : work_) {
-    w.off.release();
-    w.adj.release();
-    w.src.release();
-    w.orig.release();
-    w.sup.release();
-  }
-  for (auto& l : levels_) {
-    l->b_off.release();
-    l->b_adj.release();
-    l->b_deg.release();
-    l->b_lvl.release();
-    l->b_orig.release();
-  }
+  each_buffer([](const DevBuf& b) { const_cast<DevBuf&>(b).release(); });
   for (auto& e : ev_) (void)hipEventDestroy(e);
   if (ctl_) (void)hipFree(ctl_);
   if (pinned_) (void)hipHostFree(pinned_);

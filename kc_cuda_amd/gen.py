@@ -111,6 +111,34 @@ def batch(count: int = 4096, n_min: int = 64, n_max: int = 65536, deg_min: float
     return _take(lib, m, p), offs, ns
 
 
+def serve_query(target_edges: int, kind: str, clique: int, seed: int):
+    """One serving query of about target_edges edges: R-MAT (edge factor 16,
+    (.57,.19,.19,.05), scale chosen so that 15.3 * 2^scale is nearest the
+    target: that many edges survive deduplication) or Chung-Lu (gamma 2.1,
+    average degree 10, as many samples as target edges), with a planted
+    clique. Returns (pairs, n)."""
+    if kind == "rmat":
+        scale = int(min(28, max(5, round(np.log2(max(target_edges, 1) / 15.3)))))
+        return rmat(scale, 16, 0.57, 0.19, 0.19, True, clique, seed)
+    n = max(2 * clique, target_edges // 5)
+    return chung_lu(n, 2.1, max(target_edges, clique * clique), clique, seed)
+
+
+def serve_mix(count: int = 256, median: float = 200_000, sigma: float = 1.5,
+              cap: int = 50_000_000, seed: int = 4):
+    """Serving queries: edge counts lognormal (median, sigma) capped at cap,
+    each R-MAT or Chung-Lu (50/50) with a planted clique of U[8, 64]
+    vertices. Returns a list of (pairs, n) in arrival order."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        target = int(min(cap, max(1000, rng.lognormal(np.log(median), sigma))))
+        kind = "rmat" if rng.random() < 0.5 else "chung_lu"
+        clique = int(rng.integers(8, 65))
+        out.append(serve_query(target, kind, clique, int(rng.integers(1, 2**31 - 1))))
+    return out
+
+
 def write_tsv(path: str | os.PathLike, pairs: np.ndarray) -> int:
     """Write pairs in the reference's TSV contract; returns bytes written."""
     lib = _gen()

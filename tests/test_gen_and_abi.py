@@ -47,6 +47,22 @@ def test_batch_generator_layout_and_cliques():
             assert O.solve_pairs(q, int(ns[g])).kmax >= min(5, int(ns[g]))
 
 
+def test_serve_mix_sizes_and_layout():
+    """Serving queries: deterministic, canonical pairs, lognormal sizes near
+    the requested median, planted cliques that bound kmax from below."""
+    from kc_cuda_amd import gen
+    a = gen.serve_mix(12, 20_000, 1.0, 200_000, 5)
+    b = gen.serve_mix(12, 20_000, 1.0, 200_000, 5)
+    for (p, n), (q, m) in zip(a, b):
+        np.testing.assert_array_equal(p, q)
+        assert n == m and p.dtype == np.uint32
+        assert np.all(p[:, 0] < p[:, 1]) and np.all(p[:, 1] < n)
+    sizes = np.array([len(p) for p, _ in a])
+    assert 5_000 < np.median(sizes) < 80_000 and sizes.max() <= 300_000
+    p, n = gen.serve_query(5_000, "chung_lu", 12, 3)
+    assert O.solve_pairs(p, n).kmax >= 12
+
+
 def test_er_density_and_clique():
     from kc_cuda_amd import gen
     p, n = gen.er_clique(16384, 16.0, 24, 0)

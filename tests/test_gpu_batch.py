@@ -8,7 +8,12 @@ import pytest
 from conftest import canon_pairs
 from graphs import closed_forms
 
-pytestmark = pytest.mark.gpu
+# The persistent batched kernel raised a GPU memory fault on this file's first
+# test (seed 11) twice, with no cause shown on the CPU; its entry points now
+# fail and these tests stay skipped until the cause is found and fixed.
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="batched kernel switched off: unexplained GPU memory fault "
+                                      "on test_batch_matches_plain[11]")]
 
 
 def _with_specials(pairs, offs, ns):
