@@ -150,11 +150,10 @@ int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* 
  * served and solved concurrently on `slots` solver contexts, each with its
  * own HIP stream and host thread. Each slot keeps its CSR, support and peel
  * buffers resident between queries and is charged against hbm_budget what
- * they hold, or the footprint of its running query (kt_serve_footprint) if
- * larger. A query is admitted as soon as a slot is free and that charge fits
- * in the budget; when nothing runs and it still does not fit, idle slots
- * free their buffers. A query that could not fit even then is rejected
- * (status 1). Results equal kt_solve_pairs_* on the same graph, alone or in
+ * they hold, plus the footprint of its running query (kt_serve_footprint).
+ * A query is admitted as soon as a slot is free and that charge fits in the
+ * budget; when nothing runs and it still does not fit, idle slots free their
+ * buffers. A query that could not fit even then is rejected (status 1). Results equal kt_solve_pairs_* on the same graph, alone or in
  * the mix. */
 typedef struct kt_server kt_server;
 typedef struct kt_serve_result {

@@ -1148,10 +1148,14 @@ void core_round(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t* 
                 uint32_t* lvl, uint32_t* deg, HeavyList hl, hipStream_t s) {
   constexpr int kPushGrid = 2048;
   if (!n) return;
+  // grids sized by the vertex count: small graphs launch a few workgroups
+  const int push_grid = int(std::min<uint64_t>(kPushGrid, std::max<uint64_t>(4, (uint64_t(n) + 63) / 64)));
   k_core_sweep<<<grid_tiles(n), kBlock, 0, s>>>(lvl, deg, n, k, q, ctl);
-  k_core_push<<<kPushGrid, kBlock, 0, s>>>(off, adj, q, ctl, lvl, deg, hl.items, hl.count);
-  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices, ctl);
-  k_core_push_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, hl.items, hl.count, hl.slices, lvl, deg, ctl);
+  k_core_push<<<push_grid, kBlock, 0, s>>>(off, adj, q, ctl, lvl, deg, hl.items, hl.count);
+  if (uint64_t(n) > kHeavyRow + 1) {  // else no row can be heavy
+    k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices, ctl);
+    k_core_push_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, hl.items, hl.count, hl.slices, lvl, deg, ctl);
+  }
   k_vround_close<<<1, kWave, 0, s>>>(ctl);
 }
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, uint32_t n,
@@ -1160,8 +1164,10 @@ void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t
   fill_u32(hl.count, 1, 0, s);
   k_core_alive_deg<<<std::min(grid_for(uint64_t(n) * kWave), 8192), kBlock, 0, s>>>(off, adj, lvl, n, deg,
                                                                                     hl.items, hl.count);
-  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices, nullptr);
-  k_core_alive_deg_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, lvl, hl.items, hl.count, hl.slices, deg);
+  if (uint64_t(n) > kHeavyRow + 1) {  // else no row can be heavy
+    k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices, nullptr);
+    k_core_alive_deg_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, lvl, hl.items, hl.count, hl.slices, deg);
+  }
 }
 void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s) {
   if (n) k_core_reopen_above<<<grid_for(n), kBlock, 0, s>>>(lvl, n, k);

@@ -16,6 +16,8 @@
 // The host enqueues rounds in batches, reads the control block once per batch
 // and steps in only to drop a frontier, compact, or collect the answer; a
 // kernel whose action does not concern it does nothing.
+#include <algorithm>
+
 #include "kt_device.h"
 #include "kt_kernels.h"
 
@@ -464,9 +466,18 @@ void round_start(Ctl* ctl, uint32_t r, uint64_t alive, int32_t thr, const RoundR
   k_round_start<<<1, kWave, 0, s>>>(ctl, r, alive, thr, rule, reset ? 1u : 0u, search);
 }
 
+// Workgroups for `items` units of work at `per_block` units each, within
+// [1, cap]: a small core launches a few workgroups instead of the full grid
+// (every round kernel loops grid-stride, so any grid is correct).
+static int work_grid(uint64_t items, uint64_t per_block, int cap) {
+  const uint64_t g = (items + per_block - 1) / per_block;
+  return int(std::max<uint64_t>(1, std::min<uint64_t>(g, uint64_t(cap))));
+}
+
 static void launch_stamp(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, hipStream_t s) {
-  k_round_stamp<<<kStampGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.state, a.sup, a.orig, a.del_thr, a.src, a.dst,
-                                              reinterpret_cast<unsigned long long*>(a.fbits), a.ws, rule);
+  k_round_stamp<<<work_grid(a.mc, 4 * kBlock, kStampGrid), kBlock, 0, s>>>(
+      a.q0, a.q1, ctl, a.state, a.sup, a.orig, a.del_thr, a.src, a.dst,
+      reinterpret_cast<unsigned long long*>(a.fbits), a.ws, rule);
 }
 
 void round_stamp(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, hipStream_t s) { launch_stamp(a, ctl, rule, s); }
@@ -475,19 +486,21 @@ void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, 
   auto bits = reinterpret_cast<unsigned long long*>(a.bits);
   auto fbits = reinterpret_cast<unsigned long long*>(a.fbits);
   auto b0 = reinterpret_cast<const unsigned long long*>(a.bits0);
+  const int peel_grid = work_grid(a.mc, kBlock, kPeelGrid);
+  const int advance_grid = rule.dense ? work_grid(a.mc, 4 * kBlock, kAdvanceGrid) : 1;
   for (int i = 0; i < count; ++i) {
     if (rule.dense) {
       if (a.eidm)
-        k_dense_peel_round<true><<<kPeelGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst, bits, fbits, a.ws, a.eidm,
+        k_dense_peel_round<true><<<peel_grid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst, bits, fbits, a.ws, a.eidm,
                                                               a.nc, b0, a.pref0, a.off_u, a.eid_u, a.state, a.sup,
                                                               a.mc);
       else
-        k_dense_peel_round<false><<<kPeelGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst, bits, fbits, a.ws,
+        k_dense_peel_round<false><<<peel_grid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst, bits, fbits, a.ws,
                                                                a.eidm, a.nc, b0, a.pref0, a.off_u, a.eid_u, a.state,
                                                                a.sup, a.mc);
     } else {
 #define KT_SPARSE_ROUND(G)                                                                                 \
-  k_sparse_peel_round<G><<<kPeelGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.off_u, a.adj_u, a.eid_u, a.src, a.dst, \
+  k_sparse_peel_round<G><<<peel_grid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.off_u, a.adj_u, a.eid_u, a.src, a.dst, \
                                                       a.state, a.sup, a.mc)
       switch (lanes) {
         case 8: KT_SPARSE_ROUND(8); break;
@@ -497,7 +510,7 @@ void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, 
       }
 #undef KT_SPARSE_ROUND
     }
-    k_round_advance<<<rule.dense ? kAdvanceGrid : 1, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst,
+    k_round_advance<<<advance_grid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst,
                                                                      rule.dense ? bits : nullptr,
                                                                      rule.dense ? fbits : nullptr, a.ws, rule);
     launch_stamp(a, ctl, rule, s);
@@ -505,7 +518,7 @@ void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, 
 }
 
 void frontier_clear(const PeelArgs& a, const Ctl* ctl, hipStream_t s) {
-  k_frontier_clear<<<kClearGrid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst,
+  k_frontier_clear<<<work_grid(a.mc, kBlock, kClearGrid), kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst,
                                                  reinterpret_cast<unsigned long long*>(a.bits),
                                                  reinterpret_cast<unsigned long long*>(a.fbits), a.ws);
 }

@@ -15,12 +15,13 @@ double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_c
 // Per input edge: the pairs' device copy, the CSR and its build and sort
 // buffers, the vertex-peel nest's subgraphs, and for every core edge its
 // oriented and undirected slots with edge ids, support, state, deletion
-// level, queues, backups and both compaction sets (256 B in all); per vertex
+// level, queues, backups and both compaction sets (160 B in all; measured at
+// most 96 B on serving and batch mixes and RMAT scale 22); per vertex
 // offsets, degrees, levels and maps (64 B); a dense core's bit rows (at most
 // 72 B per edge, since a row is no longer than two sorted rows) and its
 // edge-id matrix (at most kServeEidmBytes); 16 MiB of fixed buffers.
 uint64_t Server::footprint(uint64_t m, uint32_t n) {
-  return 256ull * m + 64ull * uint64_t(n) + std::min<uint64_t>(72ull * m, kServeEidmBytes) +
+  return 160ull * m + 64ull * uint64_t(n) + std::min<uint64_t>(72ull * m, kServeEidmBytes) +
          std::min<uint64_t>(512ull * m, kServeEidmBytes) + (16ull << 20);
 }
 
@@ -126,7 +127,7 @@ void Server::run_slot(uint32_t s) {
       std::unique_lock<std::mutex> lk(mu_);
       auto fits = [&] {
         const Query& h = queue_.front();
-        return ctr_.bytes_in_use - slot.charge + std::max(slot.charge, h.bytes) <= budget_;
+        return ctr_.bytes_in_use + h.bytes <= budget_;  // the slot may grow by the footprint
       };
       cv_admit_.wait(lk, [&] {
         if (stop_) return true;
@@ -142,7 +143,8 @@ void Server::run_slot(uint32_t s) {
         queue_.pop_front();
         ctr_.in_flight += 1;
         ctr_.peak_in_flight = std::max(ctr_.peak_in_flight, ctr_.in_flight);
-        const uint64_t charge = std::max(slot.charge, q.bytes);
+        // its buffers hold slot.held now and grow by at most the query's footprint
+        const uint64_t charge = slot.held + q.bytes;
         ctr_.bytes_in_use = ctr_.bytes_in_use - slot.charge + charge;
         slot.charge = charge;
         ctr_.peak_bytes_in_use = std::max(ctr_.peak_bytes_in_use, ctr_.bytes_in_use);

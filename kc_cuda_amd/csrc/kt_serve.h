@@ -50,7 +50,8 @@ class Server {
  public:
   // slots: solver contexts (and streams); budget: HBM bytes the slots may
   // hold in total. A slot is charged what its buffers hold while idle and
-  // max(that, footprint of its query) while it runs.
+  // that plus the footprint of its query while it runs (a buffer grows only
+  // to what the query needs).
   Server(int device, int slots, uint64_t budget);
   ~Server();
   Server(const Server&) = delete;
