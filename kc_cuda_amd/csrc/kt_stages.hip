@@ -534,12 +534,14 @@ TrussOutcome Solver::run_levels(CoreView* cv, int32_t thr, uint64_t alive, uint3
 // not over by then, thresholds are probed instead: each probe peels one
 // level at t from the last non-empty state (kept in a backup) and is undone
 // when it empties the view. Thresholds gallop up by doubling steps from the
-// minimum support, then bisect. The result equals the level walk's: the last
+// minimum support, then bisect until at most kWalkWindow thresholds are left,
+// whose levels are then walked. The result equals the level walk's: the last
 // non-empty probe leaves the t-level, whose minimum support is t* (a level at
 // t* + 1 is empty), so kmax = t* + 2 with its edge count, and every edge
 // deleted on the way carries a threshold <= the floor. (The reference's
 // speculative k-steps with backups, src/kc.cu:1189-1196, 1252-1272.)
 TrussOutcome Solver::truss_gallop(CoreView* cv) {
+  constexpr int64_t kWalkWindow = 32;  // thresholds left when probing gives way to walking
   TrussOutcome out;
   if (cv->mc == 0) return out;
   SearchRule walk{};
@@ -580,6 +582,25 @@ TrussOutcome Solver::truss_gallop(CoreView* cv) {
   uint64_t alive = out.count;
   int64_t step = 1;
   while (hi < 0 || hi - lo > 1) {
+    if (hi >= 0 && hi - lo <= kWalkWindow) {
+      // A short window is left: walk its levels. A probe that empties the
+      // view costs a full cascade and is then undone; the walk's only empty
+      // level is the last one.
+      SearchRule all{};
+      const TrussOutcome w = run_levels(cv, int32_t(lo + 1), alive, &r, all, true);
+      out.rounds += w.rounds;
+      out.levels += w.levels;
+      out.compactions += w.compactions;
+      if (w.kmax >= 2 && w.count > 0) {
+        out.kmax = w.kmax;
+        out.count = w.count;
+        out.floor_thr = w.floor_thr;
+      }
+      if (trace_)
+        std::fprintf(stderr, "[kt] gallop walk from t=%lld -> kmax=%u count=%llu\n", (long long)(lo + 1), out.kmax,
+                     (unsigned long long)out.count);
+      break;
+    }
     const int64_t t = hi < 0 ? lo + step : lo + (hi - lo) / 2;
     save();
     const TrussOutcome p = run_levels(cv, int32_t(t), alive, &r, single, false);
