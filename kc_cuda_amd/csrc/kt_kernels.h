@@ -107,7 +107,7 @@ uint32_t dense_words(uint32_t nc);
 bool dense_fits(uint32_t nc, uint32_t mc);
 void dense_rows(const uint32_t* off_u, const uint32_t* adj_u, uint32_t nc, uint32_t ws,
                 uint64_t* bits0, uint64_t* bits, uint32_t* pref0, hipStream_t s);
-void dense_support(const uint32_t* src, const uint32_t* dst, uint32_t mc, const uint64_t* bits,
+void dense_support(const uint32_t* off_d, const uint32_t* dst, uint32_t nc, uint32_t mc, const uint64_t* bits,
                    uint32_t ws, const uint32_t* state, int32_t* sup, hipStream_t s);
 
 // Device-driven edge-peel rounds and truss levels (kt_peel.hip). q0/q1 are

@@ -888,23 +888,35 @@ __global__ __launch_bounds__(kBlock) void k_dense_rows(const uint32_t* __restric
 }
 
 // sup(e) = |N(u) ∩ N(v)| = popcount(bits[u] & bits[v]) for every live
-// oriented edge (all edges when state is null); G lanes per edge.
+// oriented edge (all edges when state is null). A workgroup takes one source
+// row u at a time (rows grid-stride), stages bits[u] in LDS once, and its
+// groups of G lanes walk u's out-edges: each edge then reads one row from
+// global memory (bits[v]) instead of two.
+constexpr uint32_t kDenseMaxWords = 1024;  // nc <= 65536 on a dense core
 template <int G>
-__global__ __launch_bounds__(kBlock) void k_dense_support(const uint32_t* __restrict__ src,
-                                                          const uint32_t* __restrict__ dst, uint32_t mc,
+__global__ __launch_bounds__(kBlock) void k_dense_support(const uint32_t* __restrict__ off_d,
+                                                          const uint32_t* __restrict__ dst, uint32_t nc,
                                                           const unsigned long long* __restrict__ bits,
                                                           uint32_t ws, const uint32_t* __restrict__ state,
                                                           int32_t* sup) {
-  const uint32_t groups = gridDim.x * (blockDim.x / G);
+  __shared__ unsigned long long s_row[kDenseMaxWords];
+  const uint32_t grp = threadIdx.x / G, ngrp = blockDim.x / G;
   const uint32_t lane = threadIdx.x % G;
-  for (uint32_t e = blockIdx.x * (blockDim.x / G) + threadIdx.x / G; e < mc; e += groups) {
-    if (state && state[e] != kNone) continue;  // group-uniform
-    const unsigned long long* a = bits + uint64_t(src[e]) * ws;
-    const unsigned long long* b = bits + uint64_t(dst[e]) * ws;
-    uint32_t c = 0;
-    for (uint32_t i = lane; i < ws; i += G) c += uint32_t(__popcll(a[i] & b[i]));
-    c = group_sum<G>(c);
-    if (lane == 0) sup[e] = int32_t(c);
+  for (uint32_t u = blockIdx.x; u < nc; u += gridDim.x) {  // workgroup-uniform
+    const uint32_t e0 = off_d[u], e1 = off_d[u + 1];
+    if (e0 == e1) continue;
+    __syncthreads();  // the previous row's readers are done
+    const unsigned long long* a = bits + uint64_t(u) * ws;
+    for (uint32_t i = threadIdx.x; i < ws; i += blockDim.x) s_row[i] = a[i];
+    __syncthreads();
+    for (uint32_t e = e0 + grp; e < e1; e += ngrp) {
+      if (state && state[e] != kNone) continue;  // group-uniform
+      const unsigned long long* b = bits + uint64_t(dst[e]) * ws;
+      uint32_t c = 0;
+      for (uint32_t i = lane; i < ws; i += G) c += uint32_t(__popcll(s_row[i] & b[i]));
+      c = group_sum<G>(c);
+      if (lane == 0) sup[e] = int32_t(c);
+    }
   }
 }
 
@@ -1029,15 +1041,17 @@ void dense_rows(const uint32_t* off_u, const uint32_t* adj_u, uint32_t nc, uint3
       off_u, adj_u, nc, ws, reinterpret_cast<unsigned long long*>(bits0),
       reinterpret_cast<unsigned long long*>(bits), pref0);
 }
-void dense_support(const uint32_t* src, const uint32_t* dst, uint32_t mc, const uint64_t* bits, uint32_t ws,
-                   const uint32_t* state, int32_t* sup, hipStream_t s) {
-  if (!mc) return;
+void dense_support(const uint32_t* off_d, const uint32_t* dst, uint32_t nc, uint32_t mc, const uint64_t* bits,
+                   uint32_t ws, const uint32_t* state, int32_t* sup, hipStream_t s) {
+  if (!mc || !nc) return;
+  if (ws > kDenseMaxWords) fail("dense support: rows longer than the LDS stage");
   auto b = reinterpret_cast<const unsigned long long*>(bits);
+  const int grid = int(std::min<uint32_t>(nc, 4096));
   switch (dense_lanes(ws)) {
-    case 8: k_dense_support<8><<<grid_for(uint64_t(mc) * 8), kBlock, 0, s>>>(src, dst, mc, b, ws, state, sup); break;
-    case 16: k_dense_support<16><<<grid_for(uint64_t(mc) * 16), kBlock, 0, s>>>(src, dst, mc, b, ws, state, sup); break;
-    case 32: k_dense_support<32><<<grid_for(uint64_t(mc) * 32), kBlock, 0, s>>>(src, dst, mc, b, ws, state, sup); break;
-    default: k_dense_support<64><<<grid_for(uint64_t(mc) * 64), kBlock, 0, s>>>(src, dst, mc, b, ws, state, sup); break;
+    case 8: k_dense_support<8><<<grid, kBlock, 0, s>>>(off_d, dst, nc, b, ws, state, sup); break;
+    case 16: k_dense_support<16><<<grid, kBlock, 0, s>>>(off_d, dst, nc, b, ws, state, sup); break;
+    case 32: k_dense_support<32><<<grid, kBlock, 0, s>>>(off_d, dst, nc, b, ws, state, sup); break;
+    default: k_dense_support<64><<<grid, kBlock, 0, s>>>(off_d, dst, nc, b, ws, state, sup); break;
   }
 }
 void row_src(const uint32_t* off, uint32_t rows, uint32_t* src, hipStream_t s) {

@@ -310,7 +310,7 @@ void Solver::count_support(CoreView* cv, float* ms) {
   uint32_t kind = 1;
   if (cv->dense) {
     kind = 2;
-    dense_support(cv->src, cv->adj_d, cv->mc, cv->bits, cv->ws, nullptr, cv->sup, stream_);
+    dense_support(cv->off_d, cv->adj_d, cv->nc, cv->mc, cv->bits, cv->ws, nullptr, cv->sup, stream_);
   } else if (!force_edge_count_ && support_bitmap_fits(cv->nc, cv->max_dout)) {
     kind = 0;
     support_bitmap(cv->off_d, cv->adj_d, cv->nc, cv->max_dout, lanes, cv->sup, ctl_, stream_);
@@ -506,7 +506,7 @@ TrussOutcome Solver::run_levels(CoreView* cv, int32_t thr, uint64_t alive, uint3
           r = 1;
         } else {
           frontier_clear(peel_args(*cv), ctl_, stream_);
-          dense_support(cv->src, cv->adj_d, cv->mc, cv->bits, cv->ws, cv->state, cv->sup, stream_);
+          dense_support(cv->off_d, cv->adj_d, cv->nc, cv->mc, cv->bits, cv->ws, cv->state, cv->sup, stream_);
         }
       } else {
         compact_core(cv, false);
