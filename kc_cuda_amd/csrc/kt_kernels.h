@@ -61,12 +61,19 @@ void alive_list(const uint32_t* lvl, uint32_t n, uint32_t* list, uint32_t* new_i
                 uint32_t* count, hipStream_t s);
 void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_orig, uint32_t na,
                  uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s);
-// new_id from alive_list (kNone marks removed vertices). cursor: cursor_cap
-// zeroed 64-bit slots, at least the number of rows of the cut longer than the
-// heavy threshold (slots / 8192 + 1 suffices)
-void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* new_id,
-              uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl, uint64_t* cursor,
-              uint32_t cursor_cap, hipStream_t s);
+// Alive vertices of a level as a bitmap (bits: (n + 31) / 32 words) with the
+// popcount of every word in cnt (words + 1 entries, the last set to 0); an
+// exclusive scan of cnt gives pref, and ranked_list writes the alive vertices
+// in increasing id (list[rank] = vertex).
+void alive_words(const uint32_t* lvl, uint32_t n, uint32_t* bits, uint32_t* cnt, hipStream_t s);
+void alive_ranked_list(const uint32_t* bits, const uint32_t* pref, uint32_t n, uint32_t* list, hipStream_t s);
+// Rows of the cut: each alive neighbour w becomes its rank (abits/apref from
+// alive_words and the scan). cursor: cursor_cap zeroed 64-bit slots, at least
+// the number of rows of the cut longer than the heavy threshold (slots / 8192
+// + 1 suffices)
+void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* abits,
+              const uint32_t* apref, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl,
+              uint64_t* cursor, uint32_t cursor_cap, hipStream_t s);
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
                   const uint32_t* inv, uint32_t nc, uint32_t* dout, hipStream_t s);
 void orient_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* to,

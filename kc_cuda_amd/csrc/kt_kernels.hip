@@ -504,6 +504,50 @@ __global__ __launch_bounds__(kBlock) void k_alive_list(const uint32_t* __restric
       count);
 }
 
+// Alive vertices as a bitmap (bit v % 32 of word v / 32) with per-word
+// prefix counts: the rank of an alive vertex (its id in the cut) is
+// pref[word] + the alive bits below it. The two arrays take n / 4 bytes, so
+// the rank lookups of a cut stay in L2 and the MALL where an n-entry id map
+// would be read from HBM.
+__device__ __forceinline__ uint32_t alive_rank(const uint32_t* __restrict__ bits, const uint32_t* __restrict__ pref,
+                                               uint32_t w) {
+  const uint32_t b = bits[w >> 5], m = 1u << (w & 31);
+  return (b & m) ? pref[w >> 5] + uint32_t(__popc(b & (m - 1u))) : kNone;
+}
+
+// bits[i] and cnt[i] = popcount for every word; cnt[words] = 0 (the scan's tail).
+__global__ __launch_bounds__(kBlock) void k_alive_words(const uint32_t* __restrict__ lvl, uint32_t n, uint32_t* bits,
+                                                        uint32_t* cnt) {
+  const uint32_t words = (n + 31) / 32;
+  const uint32_t groups = (words + 1) / 2;  // 64 vertices per wave step
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t g = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; g < groups; g += waves) {
+    const uint32_t v = g * 64 + uint32_t(lane);
+    const unsigned long long m = __ballot(v < n && lvl[v] == kNone);
+    const uint32_t i = g * 2 + uint32_t(lane);
+    if (lane < 2 && i < words) {
+      const uint32_t b = uint32_t(m >> (32 * lane));
+      bits[i] = b;
+      cnt[i] = uint32_t(__popc(b));
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt[words] = 0;
+}
+
+// list[rank] = vertex, ranks in increasing vertex id.
+__global__ __launch_bounds__(kBlock) void k_alive_ranked_list(const uint32_t* __restrict__ bits,
+                                                              const uint32_t* __restrict__ pref, uint32_t words,
+                                                              uint32_t* list) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) {
+    uint32_t b = bits[i], r = pref[i];
+    while (b) {
+      list[r++] = i * 32 + uint32_t(__ffs(b) - 1);
+      b &= b - 1u;
+    }
+  }
+}
+
 __global__ void k_sub_degrees(const uint32_t* __restrict__ list, const uint32_t* __restrict__ deg,
                               const uint32_t* __restrict__ to_orig, uint32_t na, uint32_t* sub_deg,
                               uint32_t* sub_orig) {
@@ -520,7 +564,8 @@ __global__ void k_sub_degrees(const uint32_t* __restrict__ list, const uint32_t*
 __global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ adj,
                                                            const uint32_t* __restrict__ list,
-                                                           const uint32_t* __restrict__ new_id,
+                                                           const uint32_t* __restrict__ abits,
+                                                           const uint32_t* __restrict__ apref,
                                                            const uint64_t* __restrict__ sub_off,
                                                            uint32_t* sub_adj, const uint32_t* heavy,
                                                            const uint32_t* heavy_count,
@@ -538,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __res
     const uint64_t e = min(off[v + 1], b + kSlice);
     for (uint64_t j0 = b + (threadIdx.x & ~63u); j0 < e; j0 += blockDim.x) {
       const uint64_t j = j0 + lane;
-      const uint32_t nid = j < e ? new_id[adj[j]] : kNone;
+      const uint32_t nid = j < e ? alive_rank(abits, apref, adj[j]) : kNone;
       const bool p = nid != kNone;
       const uint64_t mask = __ballot(p);
       if (!mask) continue;
@@ -554,7 +599,8 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __res
 __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict__ off,
                                                      const uint32_t* __restrict__ adj,
                                                      const uint32_t* __restrict__ list,
-                                                     const uint32_t* __restrict__ new_id, uint32_t na,
+                                                     const uint32_t* __restrict__ abits,
+                                                     const uint32_t* __restrict__ apref, uint32_t na,
                                                      const uint64_t* __restrict__ sub_off,
                                                      uint32_t* sub_adj, uint32_t* heavy,
                                                      uint32_t* heavy_count) {
@@ -570,7 +616,7 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict_
     }
     for (uint64_t j0 = off[v]; j0 < e; j0 += kWave) {
       const uint64_t j = j0 + lane;
-      const uint32_t nid = j < e ? new_id[adj[j]] : kNone;
+      const uint32_t nid = j < e ? alive_rank(abits, apref, adj[j]) : kNone;
       const bool p = nid != kNone;
       const uint64_t mask = __ballot(p);
       if (p) sub_adj[pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = nid;
@@ -1203,17 +1249,25 @@ void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_o
                  uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s) {
   if (na) k_sub_degrees<<<grid_for(na), kBlock, 0, s>>>(list, deg, to_orig, na, sub_deg, sub_orig);
 }
-void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* new_id, uint32_t na,
-              const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl, uint64_t* cursor, uint32_t cursor_cap,
-              hipStream_t s) {
+void alive_words(const uint32_t* lvl, uint32_t n, uint32_t* bits, uint32_t* cnt, hipStream_t s) {
+  const uint32_t words = (n + 31) / 32;
+  k_alive_words<<<grid_for(uint64_t((words + 1) / 2) * kWave), kBlock, 0, s>>>(lvl, n, bits, cnt);
+}
+void alive_ranked_list(const uint32_t* bits, const uint32_t* pref, uint32_t n, uint32_t* list, hipStream_t s) {
+  const uint32_t words = (n + 31) / 32;
+  if (words) k_alive_ranked_list<<<grid_for(words), kBlock, 0, s>>>(bits, pref, words, list);
+}
+void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* abits,
+              const uint32_t* apref, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl,
+              uint64_t* cursor, uint32_t cursor_cap, hipStream_t s) {
   if (!na) return;
   fill_u32(hl.count, 1, 0, s);
   fill_u64(cursor, cursor_cap, 0, s);
-  k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, list, new_id, na, sub_off, sub_adj,
+  k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, list, abits, apref, na, sub_off, sub_adj,
                                                                hl.items, hl.count);
   k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, list, hl.slices, nullptr);
-  k_sub_fill_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, list, new_id, sub_off, sub_adj, hl.items, hl.count,
-                                                 hl.slices, reinterpret_cast<unsigned long long*>(cursor));
+  k_sub_fill_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, list, abits, apref, sub_off, sub_adj, hl.items,
+                                                 hl.count, hl.slices, reinterpret_cast<unsigned long long*>(cursor));
 }
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to, const uint32_t* inv,
                   uint32_t nc, uint32_t* dout, hipStream_t s) {

@@ -69,11 +69,17 @@ uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
 LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
   if (levels_.size() <= depth_) levels_.emplace_back(new LevelGraph);
   LevelGraph& s = *levels_[depth_];
+  // alive bitmap | word popcounts | their exclusive scan, then the ranked list
+  const size_t words = (size_t(p.n) + 31) / 32;
   list_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
-  new_id_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
-  zero_u32(&ctl_->nalive);
-  alive_list(p.lvl, p.n, list_.as<uint32_t>(), new_id_.as<uint32_t>(), &ctl_->nalive, stream_);
-  const uint32_t na = read_u32(&ctl_->nalive);
+  new_id_.ensure(sizeof(uint32_t) * 3 * (words + 1));
+  uint32_t* abits = new_id_.as<uint32_t>();
+  uint32_t* acnt = abits + words + 1;
+  uint32_t* apref = acnt + words + 1;
+  alive_words(p.lvl, p.n, abits, acnt, stream_);
+  exclusive_scan_u32(acnt, apref, uint32_t(words + 1));
+  alive_ranked_list(abits, apref, p.n, list_.as<uint32_t>(), stream_);
+  const uint32_t na = read_u32(apref + words);
   if (na != alive) fail("vertex peel bookkeeping mismatch");
   s.b_deg.ensure(sizeof(uint32_t) * (size_t(na) + 1));
   s.b_orig.ensure(sizeof(uint32_t) * (size_t(na) + 1));
@@ -94,8 +100,8 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
   s.adj = s.b_adj.as<uint32_t>();
   const uint32_t cap = uint32_t(std::min<uint64_t>(na, p.slots / 8192 + 1));
   cursor_.ensure(sizeof(uint64_t) * (size_t(cap) + 1));
-  sub_fill(p.off, p.adj, list_.as<uint32_t>(), new_id_.as<uint32_t>(), na, s.off, s.adj, heavy(),
-           cursor_.as<uint64_t>(), cap, stream_);
+  sub_fill(p.off, p.adj, list_.as<uint32_t>(), abits, apref, na, s.off, s.adj, heavy(), cursor_.as<uint64_t>(), cap,
+           stream_);
   fill_u32(s.lvl, na, kNone, stream_);
   ++depth_;
   return s;
