@@ -13,11 +13,11 @@ self-loops dropped, randomly relabelled, seed 1 (64,152,088 edges).
 
 --mode batch: 4096 graphs (n log-uniform in [64, 65536], ER or R-MAT,
 average degree U[8, 32], planted clique U[4, 40], seed 3), resident in HBM,
-solved through the serving loop with every graph submitted at once; a step is
-one pass over all of them and value = graphs/s. Beside it, sequential plain
-solves of a size-stratified sample of the same graphs (every 32nd by size,
-device-resident) give the sequential graphs/s, and their answers must equal
-the served ones.
+solved as one batch (kt_batch_solve: the union of the graphs through one
+truss decomposition); a step is one batch solve and value = graphs/s. Beside
+it, sequential plain solves of a size-stratified sample of the same graphs
+(every 32nd by size, device-resident) give the sequential graphs/s, and their
+answers must equal the batch's.
 
 --mode serve: serving queries from host memory (lognormal edge counts, median
 200k, sigma 1.5, cap 50M; R-MAT or Chung-Lu with a planted 8..64-clique, seed
@@ -230,11 +230,11 @@ def _pct(xs, q):
 
 
 def batch_main(args) -> int:
-    """The 4096-graph mix, device-resident, solved through the serving loop
-    (closed loop: every graph submitted at once) against sequential plain
+    """The 4096-graph mix, device-resident, solved as one batch (the union of
+    its graphs through one truss decomposition) against sequential plain
     solves of a size-stratified sample."""
     import numpy as np
-    from kc_cuda_amd import Context, DeviceArray, Server, gen
+    from kc_cuda_amd import Context, DeviceArray, gen
 
     t0 = time.perf_counter()
     pairs, offs, ns = gen.batch()
@@ -243,43 +243,37 @@ def batch_main(args) -> int:
     log(f"[bench] batch: {count} graphs, {total} edges, generated in {time.perf_counter() - t0:.1f}s")
     ctx = Context(0)
     d = ctx.to_device(pairs)
-    views = [DeviceArray(ctx, d.ptr + 8 * int(offs[g]), 8 * int(offs[g + 1] - offs[g])) for g in range(count)]
-    srv = Server(0, args.slots, args.budget_gib << 30)
-
-    def one_pass():
-        tickets = [srv.submit(views[g], int(ns[g])) for g in range(count)]
-        return [srv.wait(t) for t in tickets]
-
+    run = ctx.batch(d, offs, ns)
     for i in range(args.warmup):
         t1 = time.perf_counter()
-        one_pass()
-        log(f"[bench] warmup {i}: {time.perf_counter() - t1:.2f}s")
+        run.solve()
+        log(f"[bench] warmup {i}: {time.perf_counter() - t1:.3f}s")
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        res = one_pass()
+        run.solve()
     wall = (time.perf_counter() - t1) / max(args.steps, 1)
-    bad = [r for r in res if r.status != 0]
-    if bad:
-        log(f"[bench] {len(bad)} graphs failed: {bad[0].error}")
+    res = run.result()
+    if (res.status != 0).any():
+        log("[bench] batch reported invalid graphs")
         return 3
-    counters = srv.counters()
-    log(f"[bench] served: {wall * 1e3:.1f} ms per pass, {count / wall:.0f} graphs/s, "
-        f"{args.slots} slots, counters {counters}")
+    log(f"[bench] batch: {wall * 1e3:.1f} ms per batch, {count / wall:.0f} graphs/s")
     # sequential plain solves of a size-stratified sample (every 32nd by size)
     by_size = np.argsort(-(offs[1:] - offs[:-1]).astype(np.int64), kind="stable")
     sample = by_size[::32]
     t_seq = 0.0
     for g in sample:
+        m = int(offs[g + 1] - offs[g])
+        view = DeviceArray(ctx, d.ptr + 8 * int(offs[g]), 8 * m)
         t2 = time.perf_counter()
-        r = ctx.solve_device(views[g], views[g].nbytes // 8, int(ns[g]))
+        r = ctx.solve_device(view, m, int(ns[g]))
         t_seq += time.perf_counter() - t2
-        if (r.kmax, r.truss_edges) != (res[g].kmax, res[g].truss_edges):
-            log(f"[bench] MISMATCH graph {g}: plain {r.kmax},{r.truss_edges} served "
-                f"{res[g].kmax},{res[g].truss_edges}")
+        if (r.kmax, r.truss_edges) != (int(res.kmax[g]), int(res.edges[g])):
+            log(f"[bench] MISMATCH graph {g}: plain {r.kmax},{r.truss_edges} batch "
+                f"{res.kmax[g]},{res.edges[g]}")
             return 3
     seq_gps = len(sample) / t_seq
     log(f"[bench] sequential plain: {len(sample)} sampled graphs in {t_seq:.2f}s -> {seq_gps:.1f} graphs/s")
-    srv.close()
+    run.free()
     d.free()
     ctx.close()
     line = {
@@ -290,12 +284,11 @@ def batch_main(args) -> int:
         "config": {"workload": "batch4096", "graphs": count, "edges": total,
                    "generator": "n log-uniform [64, 65536], ER or R-MAT(.57,.19,.19,.05) 50/50, "
                                 "avg degree U[8,32], clique U[4,40], seed 3",
-                   "engine": f"serving loop, {args.slots} slots, closed loop, inputs in HBM",
+                   "engine": "one truss decomposition of the union of the batch, inputs in HBM",
                    "edges_per_s": total / wall,
                    "sequential_plain_graphs_per_s": seq_gps,
                    "sequential_sample": f"{len(sample)} graphs, every 32nd by size, device-resident",
-                   "speedup_vs_sequential": (count / wall) / seq_gps,
-                   "peak_in_flight": counters["peak_in_flight"]},
+                   "speedup_vs_sequential": (count / wall) / seq_gps},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

@@ -127,20 +127,17 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
  * concatenated in HBM at d_pairs; h_offsets (count + 1 pair offsets) and h_ns
  * (vertex counts) are host arrays. Per graph: d_kmax, d_count (edges in its
  * kmax-truss) and d_status (0 ok, 1 invalid input); d_keep (nullable) gets 1
- * for every pair in its graph's kmax-truss. All graphs run in one persistent
- * launch; KT_BATCH_GRAPH replays a captured hipGraph of it while the
- * arguments stay the same. Results equal kt_solve_pairs_* on each graph.
- * Currently switched off: both batch entry points return an error (the
- * kernel raised an unexplained GPU memory fault on its parity test); the
- * kt_serve_* entry points solve many graphs concurrently instead. */
+ * for every pair in its graph's kmax-truss. The valid graphs are solved
+ * together as one disjoint union: one truss decomposition of the union on
+ * this context, from which every graph's answer is read. Results equal
+ * kt_solve_pairs_* on each graph. KT_BATCH_GRAPH is accepted and ignored. */
 #define KT_BATCH_GRAPH 1
 int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
                    const uint32_t* h_ns, uint32_t count, uint32_t* d_kmax, uint64_t* d_count,
                    uint32_t* d_status, uint8_t* d_keep, int flags);
-/* Autotuner for kt_batch_solve: times up to per_bucket graphs of every size
- * bucket (floor(log2 edges)) with each lanes-per-edge choice (4..64) and keeps
- * the fastest per bucket on this context; table41 (nullable) receives the
- * 41-entry table. The sample's results land in the given arrays. */
+/* Kept for callers of an earlier per-graph kernel that had per-size-bucket
+ * lane counts: the union solve has no such parameters, so this always
+ * returns an error. */
 int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
                       const uint32_t* h_ns, uint32_t count, uint32_t per_bucket, uint32_t* d_kmax,
                       uint64_t* d_count, uint32_t* d_status, uint8_t* table41);

@@ -279,7 +279,8 @@ class Context:
                     graph: bool = False) -> "BatchResult":
         """Solve every graph of a batch (pairs already in HBM, canonical and
         sorted within each graph; offsets: count + 1 pair offsets; ns: vertex
-        counts) in one persistent launch. graph: replay a captured hipGraph."""
+        counts): the valid graphs are solved together as one disjoint union.
+        graph is accepted for compatibility and has no effect."""
         run = self.batch(dpairs, offsets, ns, keep)
         try:
             run.solve(graph)
@@ -294,8 +295,7 @@ class Context:
 
     def batch_autotune(self, dpairs: DeviceArray, offsets: np.ndarray, ns: np.ndarray,
                        per_bucket: int = 16) -> np.ndarray:
-        """Tune lanes per edge for each size bucket on a sample of the batch;
-        returns the 41-entry table (0 = untuned bucket)."""
+        """Always raises KtError: the union solve has no per-bucket parameters."""
         out = _BatchBuffers(self, offsets, ns, False)
         table = np.zeros(41, dtype=np.uint8)
         try:

@@ -220,39 +220,22 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
   return 0;
 }
 
-// The persistent batched kernel (one workgroup per graph) is switched off:
-// its parity test raised a GPU memory fault that a CPU build of the same
-// kernel source (one workgroup as 256 host threads, address and undefined-
-// behaviour sanitizers) does not reproduce, so the cause is not known. Both
-// entry points fail with this message until it is.
-constexpr bool kBatchKernelEnabled = false;
-constexpr const char* kBatchOff =
-    "the persistent batched kernel is switched off (unexplained GPU memory fault on its parity test); "
-    "solve the graphs one by one, or concurrently with the kt_serve_* entry points";
-
 int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets, const uint32_t* h_ns,
                    uint32_t count, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status, uint8_t* d_keep,
                    int flags) {
-  if (!kBatchKernelEnabled) return set_err(std::string("kt_batch_solve: ") + kBatchOff);
+  (void)flags;  // KT_BATCH_GRAPH: the union solve has host decisions and is never captured
   if (!ctx || (count && (!d_pairs || !h_offsets || !h_ns || !d_kmax || !d_count || !d_status)))
     return set_err("kt_batch_solve: NULL argument");
-  return guarded([&] {
-    ctx->batcher().solve(d_pairs, h_offsets, h_ns, count, d_kmax, d_count, d_status, d_keep,
-                         (flags & KT_BATCH_GRAPH) != 0);
-  });
+  return guarded([&] { ctx->batcher().solve(d_pairs, h_offsets, h_ns, count, d_kmax, d_count, d_status, d_keep); });
 }
 
 int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets, const uint32_t* h_ns,
                       uint32_t count, uint32_t per_bucket, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status,
                       uint8_t* table41) {
-  if (!kBatchKernelEnabled) return set_err(std::string("kt_batch_autotune: ") + kBatchOff);
-  if (!ctx || (count && (!d_pairs || !h_offsets || !h_ns || !d_kmax || !d_count || !d_status)))
-    return set_err("kt_batch_autotune: NULL argument");
-  return guarded([&] {
-    ctx->batcher().autotune(d_pairs, h_offsets, h_ns, count, per_bucket ? per_bucket : 16, d_kmax, d_count,
-                            d_status);
-    if (table41) ctx->batcher().table(table41);
-  });
+  (void)ctx, (void)d_pairs, (void)h_offsets, (void)h_ns, (void)count, (void)per_bucket, (void)d_kmax, (void)d_count,
+      (void)d_status, (void)table41;
+  return set_err(
+      "kt_batch_autotune: the batched solve has no per-bucket parameters (it decomposes the union of the batch)");
 }
 
 struct kt_server {

@@ -89,6 +89,12 @@ class Solver {
 
   // Input already resident on the device: m canonical pairs (interleaved).
   void solve_pairs_device(const uint32_t* d_pairs, uint64_t m, uint32_t n, kt_result* res);
+  // Truss decomposition of the 2-core (a full level walk from threshold 1:
+  // no max-core pass, no early exit). On return *cv holds the base core
+  // edges (src0 / adj_d0 in rank ids, inv: rank -> input vertex id) and
+  // del_thr: an edge deleted at threshold t has truss number t + 1, an edge
+  // never deleted has the returned kmax (0 when the 2-core has no edges).
+  uint32_t decompose_pairs_device(const uint32_t* d_pairs, uint64_t m, uint32_t n, CoreView* cv);
   // Directed line list (x into row y) resident on the device.
   void solve_lines_device(const uint32_t* d_xs, const uint32_t* d_ys, uint64_t L, uint32_t n,
                           kt_result* res);

@@ -641,6 +641,27 @@ void Solver::solve_pairs_device(const uint32_t* d_pairs, uint64_t m, uint32_t n,
   solve_csr(res);
 }
 
+uint32_t Solver::decompose_pairs_device(const uint32_t* d_pairs, uint64_t m, uint32_t n, CoreView* out) {
+  KT_HIP(hipSetDevice(device_));
+  have_truss_ = false;
+  build_from_pairs(d_pairs, m, n);
+  depth_ = 1;
+  LevelGraph& g = level0();
+  degrees_from_offsets(g.off, g.n, g.deg, stream_);
+  fill_u32(g.lvl, g.n, kNone, stream_);
+  if (g.n) peel_vertices(g, 2);  // a vertex of degree < 2 is in no triangle
+  CoreView cv;
+  extract_core(g, &cv);
+  uint32_t kmax = 0;
+  if (cv.mc) {
+    count_support(&cv, nullptr);
+    kmax = truss_search(&cv, 1, 0, 0).kmax;
+  }
+  KT_HIP(hipStreamSynchronize(stream_));
+  *out = cv;
+  return kmax;
+}
+
 void Solver::solve_lines_device(const uint32_t* d_xs, const uint32_t* d_ys, uint64_t L, uint32_t n,
                                 kt_result* res) {
   KT_HIP(hipSetDevice(device_));
