@@ -80,7 +80,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_truss_max(const uint32_t* __re
                                                             uint32_t kall, uint32_t* gk) {
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < mc0; e += gridDim.x * blockDim.x) {
     const uint32_t g = graph_of(vstart, count, inv[src0[e]]);
-    atomicMax(&gk[g], truss_of(del_thr, e, kall));
+    const uint32_t t = truss_of(del_thr, e, kall);
+    // a graph's edges all hit one counter: most see it already at least t
+    // (the maximum only grows, so a stale read costs one extra atomic)
+    if (t > __atomic_load_n(&gk[g], __ATOMIC_RELAXED)) atomicMax(&gk[g], t);
   }
 }
 
