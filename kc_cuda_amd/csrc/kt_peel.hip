@@ -364,7 +364,9 @@ __global__ __launch_bounds__(kBlock) void k_sparse_peel_round(
   for (uint32_t s = 0; s < steps; ++s) {
     const uint32_t qi = s * groups + blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
     uint32_t ab = 0, ae = 0, bb = 0, be = 0, e = 0;
-    if (qi < cnt) {
+    // A frontier edge with no live triangle (support 0; frontier edges are
+    // not decremented within their round) has nobody to charge: no walk.
+    if (qi < cnt && sup[q[qi]] > 0) {
       e = q[qi];
       const uint32_t u = src[e], v = dst[e];
       ab = off_u[u], ae = off_u[u + 1];
