@@ -21,9 +21,11 @@ answers must equal the batch's.
 
 --mode serve: serving queries from host memory (lognormal edge counts, median
 200k, sigma 1.5, cap 50M; R-MAT or Chung-Lu with a planted 8..64-clique, seed
-4): closed-loop capacity with one slot and with --slots slots (answers must
-agree query by query), then Poisson arrivals at --load times that capacity;
-value = graphs/s of the Poisson run, with p50/p99 latency (submit to answer).
+4): the sequential rate (stand-alone solves one after another), closed-loop
+capacity with --slots slots (answers must equal the stand-alone ones query by
+query; small queries queued behind each other are grouped into union solves),
+then Poisson arrivals at --load times that capacity; value = graphs/s of the
+Poisson run, with p50/p99 latency (submit to answer).
 
 Prints one JSON line on stdout (everything else goes to stderr):
   value        edges/s = m / wall time per solve
@@ -116,6 +118,8 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=8, help="serving slots (batch and serve modes)")
     ap.add_argument("--budget-gib", type=int, default=96, help="serving HBM budget")
     ap.add_argument("--queries", type=int, default=160, help="serve mode: queries in the mix")
+    ap.add_argument("--median", type=int, default=200_000, help="serve mode: median edges per query")
+    ap.add_argument("--sigma", type=float, default=1.5, help="serve mode: lognormal sigma of the sizes")
     ap.add_argument("--load", type=float, default=0.7,
                     help="serve mode: Poisson rate as a fraction of the closed-loop capacity")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -296,17 +300,19 @@ def batch_main(args) -> int:
 
 
 def serve_main(args) -> int:
-    """Serving queries (lognormal sizes, median 200k edges, sigma 1.5, cap
-    50M; R-MAT or Chung-Lu with planted cliques of 8-64, seed 4) from host
-    memory: closed-loop capacity with 1 slot and with --slots slots (their
-    answers must agree), then Poisson arrivals at --load times that capacity
+    """Serving queries (lognormal sizes, median --median edges (200k),
+    sigma --sigma (1.5), cap 50M; R-MAT or Chung-Lu with planted cliques of
+    8-64, seed 4) from host
+    memory: the sequential rate (one stand-alone solve after another on one
+    context), closed-loop capacity with --slots slots (every answer must equal
+    its stand-alone one), then Poisson arrivals at --load times that capacity
     for p50/p99 latency."""
     import threading
     import numpy as np
-    from kc_cuda_amd import Server, gen
+    from kc_cuda_amd import Context, Server, gen
 
     t0 = time.perf_counter()
-    mix = gen.serve_mix(args.queries)
+    mix = gen.serve_mix(args.queries, args.median, args.sigma)
     edges = sum(len(p) for p, _ in mix)
     sizes = np.array([len(p) for p, _ in mix])
     log(f"[bench] serve: {len(mix)} queries, {edges} edges (median {int(np.median(sizes))}, "
@@ -319,19 +325,25 @@ def serve_main(args) -> int:
         return time.perf_counter() - t1, res
 
     budget = args.budget_gib << 30
-    with Server(0, 1, budget) as one:
-        closed_loop(one)  # warm: buffers grow to the largest query
-        t_one, res_one = closed_loop(one)
-    log(f"[bench] 1 slot: {len(mix) / t_one:.1f} queries/s")
+    with Context(0) as ctx:
+        for p, n in mix:  # warm: buffers grow to the largest query
+            ctx.solve_pairs(p, n)
+        t1 = time.perf_counter()
+        alone = [ctx.solve_pairs(p, n) for p, n in mix]
+        t_one = time.perf_counter() - t1
+    log(f"[bench] sequential: {len(mix) / t_one:.1f} queries/s")
     with Server(0, args.slots, budget) as srv:
         closed_loop(srv)
+        c0 = srv.counters()
         t_many, res_many = closed_loop(srv)
+        c1 = srv.counters()
         cap = len(mix) / t_many
-        log(f"[bench] {args.slots} slots: {cap:.1f} queries/s closed loop")
-        for i, (a, b) in enumerate(zip(res_one, res_many)):
-            if a.status or b.status or (a.kmax, a.truss_edges) != (b.kmax, b.truss_edges):
+        log(f"[bench] {args.slots} slots: {cap:.1f} queries/s closed loop, "
+            f"{c1['grouped'] - c0['grouped']} of {len(mix)} grouped")
+        for i, (a, b) in enumerate(zip(alone, res_many)):
+            if b.status or (a.kmax, a.truss_edges) != (b.kmax, b.truss_edges):
                 log(f"[bench] MISMATCH query {i}: alone {a.kmax},{a.truss_edges} in the mix "
-                    f"{b.kmax},{b.truss_edges} ({a.error}{b.error})")
+                    f"{b.kmax},{b.truss_edges} ({b.error})")
                 return 3
         rate = args.load * cap
         rng = np.random.default_rng(4)
@@ -355,6 +367,11 @@ def serve_main(args) -> int:
         res = [srv.wait(t) for t in tickets]
         wall = time.perf_counter() - t1
         counters = srv.counters()
+        for i, (a, b) in enumerate(zip(alone, res)):
+            if b.status or (a.kmax, a.truss_edges) != (b.kmax, b.truss_edges):
+                log(f"[bench] MISMATCH query {i} (Poisson): alone {a.kmax},{a.truss_edges} in the mix "
+                    f"{b.kmax},{b.truss_edges} ({b.error})")
+                return 3
     lat = [r.total_ms for r in res]
     run = [r.run_ms for r in res]
     line = {
@@ -362,14 +379,16 @@ def serve_main(args) -> int:
         "steps": 1, "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True,
         "scaling": "strong", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": "serve", "queries": len(mix), "edges": int(edges),
-                   "generator": "lognormal edges (median 200k, sigma 1.5, cap 50M), R-MAT or "
+                   "generator": f"lognormal edges (median {args.median}, sigma {args.sigma}, cap 50M), R-MAT or "
                                 "Chung-Lu 50/50, clique U[8,64], seed 4; Poisson arrivals",
                    "slots": args.slots, "hbm_budget_gib": args.budget_gib,
                    "arrival_rate_per_s": rate, "load": args.load,
                    "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
                    "p50_run_ms": _pct(run, 50), "p99_run_ms": _pct(run, 99),
-                   "capacity_graphs_per_s": cap, "one_slot_graphs_per_s": len(mix) / t_one,
-                   "speedup_vs_one_slot": t_one / t_many,
+                   "capacity_graphs_per_s": cap, "sequential_graphs_per_s": len(mix) / t_one,
+                   "capacity_vs_sequential": t_one / t_many,
+                   "grouped_closed_loop": c1["grouped"] - c0["grouped"],
+                   "grouped_poisson": counters["grouped"] - c1["grouped"],
                    "peak_in_flight": counters["peak_in_flight"],
                    "peak_gib_charged": counters["peak_bytes_in_use"] / 2**30,
                    "over_estimate": counters["over_estimate"]},

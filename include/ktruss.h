@@ -150,8 +150,13 @@ int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* 
  * they hold, plus the footprint of its running query (kt_serve_footprint).
  * A query is admitted as soon as a slot is free and that charge fits in the
  * budget; when nothing runs and it still does not fit, idle slots free their
- * buffers. A query that could not fit even then is rejected (status 1). Results equal kt_solve_pairs_* on the same graph, alone or in
- * the mix. */
+ * buffers. A query that could not fit even then is rejected (status 1).
+ * Continuous batching: when more small queries wait than idle slots can
+ * take, a slot takes several (each at most 2^16 edges, up to 64 queries and
+ * 2^22 edges in all, each one charged its footprint) and solves them as one
+ * disjoint union, as kt_batch_solve does. kmax and truss_edges equal
+ * kt_solve_pairs_* on the same graph, alone or in the mix; a grouped query's
+ * result carries n, m, kmax and truss_edges, its other fields are 0. */
 typedef struct kt_server kt_server;
 typedef struct kt_serve_result {
   int32_t status;         /* 0 solved, 1 rejected by admission, 2 solve failed */
@@ -163,6 +168,7 @@ typedef struct kt_serve_result {
   double queued_ms;       /* submit -> admitted */
   double run_ms;          /* admitted -> solved, upload included */
   double total_ms;        /* submit -> solved */
+  uint32_t group;         /* queries solved with it as one disjoint union (1: alone) */
 } kt_serve_result;
 typedef struct kt_serve_counters {
   uint64_t submitted, completed, rejected, failed;
@@ -170,6 +176,7 @@ typedef struct kt_serve_counters {
   uint64_t bytes_in_use, peak_bytes_in_use;
   uint64_t over_estimate; /* solves whose slot grew past the bytes charged for them */
   uint64_t trims;         /* idle slots that freed their buffers to let a query in */
+  uint64_t grouped;       /* queries solved in a union with others */
 } kt_serve_counters;
 int kt_serve_create(int device, int slots, uint64_t hbm_budget, kt_server** out);
 int kt_serve_destroy(kt_server* srv);

@@ -71,13 +71,13 @@ class KtServeResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("slot", ctypes.c_uint32), ("result", KtResult),
                 ("footprint", ctypes.c_uint64), ("charged", ctypes.c_uint64),
                 ("slot_bytes", ctypes.c_uint64), ("queued_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double)]
+                ("total_ms", ctypes.c_double), ("group", ctypes.c_uint32)]
 
 
 class KtServeCounters(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "submitted", "completed", "rejected", "failed", "in_flight", "peak_in_flight",
-        "bytes_in_use", "peak_bytes_in_use", "over_estimate", "trims")]
+        "bytes_in_use", "peak_bytes_in_use", "over_estimate", "trims", "grouped")]
 
     def as_dict(self) -> dict:
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
@@ -387,6 +387,7 @@ class ServeResult:
     run_ms: float
     total_ms: float
     error: str
+    group: int = 1       # queries solved with it as one disjoint union (1: alone)
 
     @property
     def kmax(self) -> int:
@@ -401,8 +402,10 @@ class Server:
     """Serving loop over the native library: graphs are queued first come,
     first served and solved concurrently on `slots` solver contexts (one HIP
     stream and host thread each); a query is admitted once a slot is free and
-    its charge (the larger of its footprint and what the slot holds) fits in
-    `hbm_budget` bytes beside the other slots' charges.
+    its charge (what the slot holds plus the query's footprint) fits in
+    `hbm_budget` bytes beside the other slots' charges. When more small
+    queries wait than idle slots can take, a slot takes several and solves
+    them as one disjoint union (continuous batching; ServeResult.group).
 
         with Server(slots=8, hbm_budget=64 << 30) as srv:
             t = srv.submit(pairs, n)
@@ -445,7 +448,7 @@ class Server:
         msg = self._lib.kt_last_error() if out.status != 0 else None
         return ServeResult(ticket, int(out.status), int(out.slot), out.result, int(out.footprint),
                            int(out.charged), int(out.slot_bytes), float(out.queued_ms), float(out.run_ms),
-                           float(out.total_ms), msg.decode() if msg else "")
+                           float(out.total_ms), msg.decode() if msg else "", int(out.group))
 
     def counters(self) -> dict:
         c = KtServeCounters()

@@ -15,8 +15,17 @@ BatchRunner::BatchRunner(Solver* s) : solver_(s) {}
 BatchRunner::~BatchRunner() {
   (void)hipSetDevice(solver_->device());
   (void)hipStreamSynchronize(solver_->stream());
-  DevBuf* all[] = {&poff_, &ns_, &status_, &vstart_, &upoff_, &union_, &gk_, &gcnt_};
-  for (auto* b : all) b->release();
+  release_buffers();
+}
+
+uint64_t BatchRunner::device_bytes() const {
+  uint64_t b = 0;
+  for (const DevBuf* d : {&poff_, &ns_, &status_, &vstart_, &upoff_, &union_, &gk_, &gcnt_}) b += d->bytes;
+  return b;
+}
+
+void BatchRunner::release_buffers() {
+  for (DevBuf* d : {&poff_, &ns_, &status_, &vstart_, &upoff_, &union_, &gk_, &gcnt_}) d->release();
 }
 
 void BatchRunner::solve(const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns, uint32_t count,

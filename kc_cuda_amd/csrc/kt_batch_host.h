@@ -18,6 +18,9 @@ class BatchRunner {
   // kmax-truss.
   void solve(const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns, uint32_t count,
              uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status, uint8_t* d_keep);
+  // device bytes of its own buffers (the solver's are counted by the solver)
+  uint64_t device_bytes() const;
+  void release_buffers();
 
  private:
   Solver* solver_;

@@ -281,6 +281,7 @@ int kt_serve_wait(kt_server* srv, uint64_t ticket, kt_serve_result* out) {
     out->queued_ms = o.queued_ms;
     out->run_ms = o.run_ms;
     out->total_ms = o.total_ms;
+    out->group = o.group;
   });
   if (rc == 0 && !query_err.empty()) g_err = query_err;
   return rc;
@@ -300,6 +301,7 @@ int kt_serve_counters_get(kt_server* srv, kt_serve_counters* out) {
     out->peak_bytes_in_use = c.peak_bytes_in_use;
     out->over_estimate = c.over_estimate;
     out->trims = c.trims;
+    out->grouped = c.grouped;
   });
 }
 

@@ -1,7 +1,8 @@
 """Serving loop: every query's kmax and |T| equal its stand-alone solve when it
-runs in a mix of concurrent queries; admission keeps the bytes charged to the
-slots within the HBM budget, rejects what can never fit, and no slot grows
-past what its query was charged (the footprint model is an upper bound)."""
+runs in a mix of concurrent queries, grouped into a union with other queries
+or not; admission keeps the bytes charged to the slots within the HBM budget,
+rejects what can never fit, and no slot grows past what its queries were
+charged (the footprint model is an upper bound)."""
 import numpy as np
 import pytest
 
@@ -42,8 +43,48 @@ def test_serve_results_equal_alone(ctx, slots):
         assert (r.kmax, r.truss_edges) == (k, e), f"query {i}"
         assert r.slot_bytes <= r.charged, f"query {i}: slot holds {r.slot_bytes} > charged {r.charged}"
     assert c["completed"] == len(qs) and c["failed"] == 0 and c["over_estimate"] == 0
-    assert c["peak_in_flight"] <= slots
     assert c["in_flight"] == 0 and c["bytes_in_use"] <= c["peak_bytes_in_use"]
+    assert c["grouped"] == sum(r.group > 1 for r in res)
+    if slots == 1:  # everything behind the first query waits: small ones are grouped
+        assert c["grouped"] > 0
+        assert c["peak_in_flight"] > 1
+
+
+def test_serve_group_falls_back_per_query(ctx):
+    """A group that holds inputs the batched solve does not take (pairs out of
+    order, pairs with u > v, an id >= n) solves those queries alone: they get
+    their stand-alone answers or error, the rest of the group its union
+    answers."""
+    from kc_cuda_amd import Server, gen
+    rng = np.random.default_rng(5)
+    good = [gen.serve_query(t, "chung_lu", 12, s) for t, s in ((40_000, 1), (60_000, 2), (20_000, 3))]
+    shuffled, sn = gen.serve_query(50_000, "chung_lu", 14, 4)
+    shuffled = np.ascontiguousarray(shuffled[rng.permutation(len(shuffled))])
+    flipped, fn = gen.serve_query(30_000, "rmat", 10, 5)
+    flipped = np.ascontiguousarray(flipped[:, ::-1])
+    bad, _ = gen.serve_query(30_000, "chung_lu", 10, 6)
+    empty = (np.zeros((0, 2), np.uint32), 10)
+    qs = [good[0], (shuffled, sn), good[1], (flipped, fn), (bad, int(bad.max())), empty, good[2]]
+    bad_i = 4
+    assert all(len(p) <= 1 << 16 for p, _ in qs)  # each small enough to join a group
+    ref = {i: (r.kmax, r.truss_edges) for i, r in
+           ((i, ctx.solve_pairs(p, n)) for i, (p, n) in enumerate(qs) if i != bad_i)}
+    with Server(0, 1, 16 << 30) as srv:
+        blocker, blk_n = gen.serve_query(3_000_000, "chung_lu", 20, 7)
+        tb = srv.submit(blocker, blk_n)  # too big to group; holds the slot while the rest queue up
+        tickets = [srv.submit(p, n) for p, n in qs]
+        res = [srv.wait(t) for t in tickets]
+        rb = srv.wait(tb)
+        c = srv.counters()
+    assert rb.status == 0 and rb.group == 1
+    for i, r in enumerate(res):
+        if i == bad_i:
+            assert r.status == 2 and "vertex id" in r.error
+            continue
+        assert r.status == 0, (i, r.error)
+        assert (r.kmax, r.truss_edges) == ref[i], f"query {i}"
+    assert {i for i, r in enumerate(res) if r.group > 1} == {0, 2, 5, 6}
+    assert c["grouped"] == 4 and c["failed"] == 1
 
 
 def test_serve_admission_by_footprint(ctx):
