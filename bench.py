@@ -122,6 +122,8 @@ def main() -> int:
     ap.add_argument("--sigma", type=float, default=1.5, help="serve mode: lognormal sigma of the sizes")
     ap.add_argument("--load", type=float, default=0.7,
                     help="serve mode: Poisson rate as a fraction of the closed-loop capacity")
+    ap.add_argument("--autotune", type=int, default=8,
+                    help="batch mode: autotune on every N-th graph before timing (0: off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=120.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -236,7 +238,8 @@ def _pct(xs, q):
 def batch_main(args) -> int:
     """The 4096-graph mix, device-resident, solved as one batch (the union of
     its graphs through one truss decomposition) against sequential plain
-    solves of a size-stratified sample."""
+    solves of a size-stratified sample. Before the warmup the autotuner
+    picks the solver settings on every --autotune-th graph (untimed)."""
     import numpy as np
     from kc_cuda_amd import Context, DeviceArray, gen
 
@@ -248,6 +251,12 @@ def batch_main(args) -> int:
     ctx = Context(0)
     d = ctx.to_device(pairs)
     run = ctx.batch(d, offs, ns)
+    tuning = None
+    if args.autotune:
+        t1 = time.perf_counter()
+        _, tuning = ctx.batch_autotune(d, offs, ns, sample_every=args.autotune)
+        log(f"[bench] autotune on every {args.autotune}th graph: {tuning} "
+            f"({time.perf_counter() - t1:.1f}s)")
     for i in range(args.warmup):
         t1 = time.perf_counter()
         run.solve()
@@ -261,6 +270,7 @@ def batch_main(args) -> int:
         log("[bench] batch reported invalid graphs")
         return 3
     log(f"[bench] batch: {wall * 1e3:.1f} ms per batch, {count / wall:.0f} graphs/s")
+    ctx.tune(0, 0, False)  # the plain solves run with the default settings
     # sequential plain solves of a size-stratified sample (every 32nd by size)
     by_size = np.argsort(-(offs[1:] - offs[:-1]).astype(np.int64), kind="stable")
     sample = by_size[::32]
@@ -292,7 +302,8 @@ def batch_main(args) -> int:
                    "edges_per_s": total / wall,
                    "sequential_plain_graphs_per_s": seq_gps,
                    "sequential_sample": f"{len(sample)} graphs, every 32nd by size, device-resident",
-                   "speedup_vs_sequential": (count / wall) / seq_gps},
+                   "speedup_vs_sequential": (count / wall) / seq_gps,
+                   "autotune": tuning},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

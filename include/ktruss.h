@@ -112,7 +112,8 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
  * bit-matrix core representation (default: by density), KT_FORCE_LEVEL_WALK
  * the level-by-level first pass on bit-matrix cores (default: 16 levels are
  * walked, then thresholds are probed), KT_FORCE_PROBES threshold probes right
- * after the first level. Every variant computes the same result. */
+ * after the first level. Every variant computes the same result. The
+ * sparse-peel drop ratio returns to its default (KT_SPARSE_DROP_RATIO or 3). */
 #define KT_FORCE_EDGE_COUNT 1
 #define KT_FORCE_ATOMIC_BUILD 2
 #define KT_FORCE_SPARSE 4
@@ -135,12 +136,26 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
 int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
                    const uint32_t* h_ns, uint32_t count, uint32_t* d_kmax, uint64_t* d_count,
                    uint32_t* d_status, uint8_t* d_keep, int flags);
-/* Kept for callers of an earlier per-graph kernel that had per-size-bucket
- * lane counts: the union solve has no such parameters, so this always
- * returns an error. */
+/* Autotuner of the batched solve (no counterpart in the reference): times
+ * kt_batch_solve on every sample_every-th graph of the batch (1: all) under
+ * candidate settings, one at a time: sparse-peel lanes, support-count lanes,
+ * support-count kernel, drop ratio (a candidate is kept when it is 3%
+ * faster than the best so far). Every trial must give every graph the same
+ * answer (else an error is returned). The fastest settings stay on the
+ * context for later solves (kt_set_tuning resets them) and the whole batch is
+ * then solved into d_kmax / d_count / d_status as kt_batch_solve does. */
+typedef struct kt_batch_tuning {
+  int32_t count_lanes;  /* support count: sub-wavefront lanes per row (0 automatic) */
+  int32_t peel_lanes;   /* sparse peel rounds: the same */
+  int32_t edge_count;   /* 1: per-edge support kernel (as KT_FORCE_EDGE_COUNT) */
+  uint32_t trials;      /* batch solves timed */
+  double drop_ratio;    /* sparse peel: recount when frontier * ratio >= live edges */
+  double ms_default;    /* sampled batch time with the context's settings on entry */
+  double ms_best;       /* with the chosen settings */
+} kt_batch_tuning;
 int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
-                      const uint32_t* h_ns, uint32_t count, uint32_t per_bucket, uint32_t* d_kmax,
-                      uint64_t* d_count, uint32_t* d_status, uint8_t* table41);
+                      const uint32_t* h_ns, uint32_t count, uint32_t sample_every, uint32_t* d_kmax,
+                      uint64_t* d_count, uint32_t* d_status, kt_batch_tuning* out);
 
 /* Serving (no counterpart in the reference, which answers one graph per
  * process, src/kc_gpu.cpp:209-347): queries are queued first come, first

@@ -74,6 +74,16 @@ class KtServeResult(ctypes.Structure):
                 ("total_ms", ctypes.c_double), ("group", ctypes.c_uint32)]
 
 
+class KtBatchTuning(ctypes.Structure):
+    _fields_ = [("count_lanes", ctypes.c_int32), ("peel_lanes", ctypes.c_int32),
+                ("edge_count", ctypes.c_int32), ("trials", ctypes.c_uint32),
+                ("drop_ratio", ctypes.c_double), ("ms_default", ctypes.c_double),
+                ("ms_best", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class KtServeCounters(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "submitted", "completed", "rejected", "failed", "in_flight", "peak_in_flight",
@@ -116,7 +126,7 @@ def native_library() -> ctypes.CDLL:
         "kt_last_csr": [_V, _V, _V],
         "kt_last_stats": [_V, ctypes.POINTER(KtStats)],
         "kt_batch_solve": [_V, _V, _V, _V, _U32, _V, _V, _V, _V, ctypes.c_int],
-        "kt_batch_autotune": [_V, _V, _V, _V, _U32, _U32, _V, _V, _V, _V],
+        "kt_batch_autotune": [_V, _V, _V, _V, _U32, _U32, _V, _V, _V, ctypes.POINTER(KtBatchTuning)],
         "kt_serve_create": [ctypes.c_int, ctypes.c_int, _U64, ctypes.POINTER(_V)],
         "kt_serve_destroy": [_V],
         "kt_serve_submit": [_V, _V, _U64, _U32, ctypes.c_int, ctypes.POINTER(_U64)],
@@ -294,18 +304,21 @@ class Context:
         return BatchRun(self, dpairs, offsets, ns, keep)
 
     def batch_autotune(self, dpairs: DeviceArray, offsets: np.ndarray, ns: np.ndarray,
-                       per_bucket: int = 16) -> np.ndarray:
-        """Always raises KtError: the union solve has no per-bucket parameters."""
+                       sample_every: int = 1) -> tuple["BatchResult", dict]:
+        """Times the batched solve of every sample_every-th graph under
+        candidate settings (peel lanes, count lanes, count kernel, drop
+        ratio), keeps the fastest on this context and returns the whole
+        batch's results with the chosen settings and timings."""
         out = _BatchBuffers(self, offsets, ns, False)
-        table = np.zeros(41, dtype=np.uint8)
+        t = KtBatchTuning()
         try:
             _check(self._lib.kt_batch_autotune(self._h, dpairs.ptr, out.offsets.ctypes.data,
-                                               out.ns.ctypes.data, out.count, per_bucket,
+                                               out.ns.ctypes.data, out.count, sample_every,
                                                out.kmax.ptr, out.edges.ptr, out.status.ptr,
-                                               table.ctypes.data))
+                                               ctypes.byref(t)))
+            return out.fetch(), t.as_dict()
         finally:
             out.free()
-        return table
 
     def alloc(self, nbytes: int) -> DeviceArray:
         p = _V()

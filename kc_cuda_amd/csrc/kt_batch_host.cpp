@@ -3,6 +3,7 @@
 #include "kt_batch_host.h"
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "kt_batch_union.h"
@@ -30,6 +31,83 @@ void BatchRunner::release_buffers() {
 
 void BatchRunner::solve(const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns, uint32_t count,
                         uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status, uint8_t* d_keep) {
+  solve_impl(d_pairs, h_off, h_ns, count, d_kmax, d_count, d_status, d_keep, 1);
+}
+
+BatchTuning BatchRunner::autotune(const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns, uint32_t count,
+                                  uint32_t every, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status) {
+  if (every == 0) fail("batch autotune: sample_every must be at least 1");
+  const hipStream_t s = solver_->stream();
+  const Solver::Tuning entry = solver_->tuning();
+  BatchTuning out;
+  std::vector<uint32_t> k0(count), s0(count), k1(count), s1(count);
+  std::vector<uint64_t> c0(count), c1(count);
+  auto trial = [&](const Solver::Tuning& cand) {
+    solver_->set_tuning(cand);
+    const auto t0 = std::chrono::steady_clock::now();
+    solve_impl(d_pairs, h_off, h_ns, count, d_kmax, d_count, d_status, nullptr, every);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (count) {
+      KT_HIP(hipMemcpyAsync(k1.data(), d_kmax, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s));
+      KT_HIP(hipMemcpyAsync(c1.data(), d_count, sizeof(uint64_t) * count, hipMemcpyDeviceToHost, s));
+      KT_HIP(hipMemcpyAsync(s1.data(), d_status, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s));
+      KT_HIP(hipStreamSynchronize(s));
+    }
+    if (out.trials == 0) {
+      k0.swap(k1), c0.swap(c1), s0.swap(s1);
+    } else if (k1 != k0 || c1 != c0 || s1 != s0) {
+      fail("batch autotune: two settings gave a graph different answers");
+    }
+    ++out.trials;
+    return ms;
+  };
+  try {
+    Solver::Tuning best = entry;
+    trial(best);  // buffers grow to the batch
+    double best_ms = trial(best);
+    out.ms_default = best_ms;
+    auto consider = [&](const Solver::Tuning& cand) {
+      const double ms = trial(cand);
+      if (ms < 0.97 * best_ms) best_ms = ms, best = cand;
+    };
+    const int widths[] = {8, 16, 32, 64};
+    const Solver::Tuning base1 = best;
+    for (int l : widths)
+      if (l != base1.peel_lanes) {
+        Solver::Tuning c = base1;
+        c.peel_lanes = l;
+        consider(c);
+      }
+    const Solver::Tuning base2 = best;
+    for (int l : widths)
+      if (l != base2.count_lanes) {
+        Solver::Tuning c = base2;
+        c.count_lanes = l;
+        consider(c);
+      }
+    Solver::Tuning c3 = best;
+    c3.edge_count = !c3.edge_count;
+    consider(c3);
+    const Solver::Tuning base4 = best;
+    for (double f : {0.5, 2.0, 4.0}) {
+      Solver::Tuning c = base4;
+      c.drop_ratio = base4.drop_ratio * f;
+      consider(c);
+    }
+    solver_->set_tuning(best);
+    out.best = best;
+    out.ms_best = best_ms;
+    solve_impl(d_pairs, h_off, h_ns, count, d_kmax, d_count, d_status, nullptr, 1);
+  } catch (...) {
+    solver_->set_tuning(entry);
+    throw;
+  }
+  return out;
+}
+
+void BatchRunner::solve_impl(const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns, uint32_t count,
+                             uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status, uint8_t* d_keep,
+                             uint32_t every) {
   KT_HIP(hipSetDevice(solver_->device()));
   if (!count) return;
   const hipStream_t s = solver_->stream();
@@ -47,6 +125,11 @@ void BatchRunner::solve(const uint32_t* d_pairs, const uint64_t* h_off, const ui
   std::vector<uint32_t> st(count);
   KT_HIP(hipMemcpyAsync(st.data(), status_.p, sizeof(uint32_t) * count, hipMemcpyDeviceToHost, s));
   KT_HIP(hipStreamSynchronize(s));
+  if (every > 1) {  // a sample: the other graphs take no part
+    for (uint32_t g = 0; g < count; ++g)
+      if (g % every) st[g] = 2;
+    KT_HIP(hipMemcpyAsync(status_.p, st.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice, s));
+  }
 
   // union numbering of the valid graphs (an invalid graph takes no vertex)
   std::vector<uint32_t> vstart(size_t(count) + 1);

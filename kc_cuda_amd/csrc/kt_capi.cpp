@@ -216,6 +216,7 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
   ctx->solver->set_force_atomic_build((flags & KT_FORCE_ATOMIC_BUILD) != 0);
   ctx->solver->set_core_mode((flags & KT_FORCE_SPARSE) ? 1 : (flags & KT_FORCE_DENSE) ? 2 : 0);
   ctx->solver->set_gallop((flags & KT_FORCE_LEVEL_WALK) == 0, (flags & KT_FORCE_PROBES) ? 1u : 16u);
+  ctx->solver->reset_drop_ratio();
   g_err.clear();
   return 0;
 }
@@ -230,12 +231,21 @@ int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_o
 }
 
 int kt_batch_autotune(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets, const uint32_t* h_ns,
-                      uint32_t count, uint32_t per_bucket, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status,
-                      uint8_t* table41) {
-  (void)ctx, (void)d_pairs, (void)h_offsets, (void)h_ns, (void)count, (void)per_bucket, (void)d_kmax, (void)d_count,
-      (void)d_status, (void)table41;
-  return set_err(
-      "kt_batch_autotune: the batched solve has no per-bucket parameters (it decomposes the union of the batch)");
+                      uint32_t count, uint32_t sample_every, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status,
+                      kt_batch_tuning* out) {
+  if (!ctx || !out || (count && (!d_pairs || !h_offsets || !h_ns || !d_kmax || !d_count || !d_status)))
+    return set_err("kt_batch_autotune: NULL argument");
+  return guarded([&] {
+    const kt::BatchTuning t =
+        ctx->batcher().autotune(d_pairs, h_offsets, h_ns, count, sample_every, d_kmax, d_count, d_status);
+    out->count_lanes = t.best.count_lanes;
+    out->peel_lanes = t.best.peel_lanes;
+    out->edge_count = t.best.edge_count ? 1 : 0;
+    out->trials = t.trials;
+    out->drop_ratio = t.best.drop_ratio;
+    out->ms_default = t.ms_default;
+    out->ms_best = t.ms_best;
+  });
 }
 
 struct kt_server {

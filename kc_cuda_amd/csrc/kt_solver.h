@@ -105,6 +105,20 @@ class Solver {
   // Stage access for parity tests: device CSR of the last solve.
   void csr_to_host(uint64_t* off, uint32_t* adj);
   void set_lanes(int count_lanes, int peel_lanes) { lanes_count_ = count_lanes; lanes_peel_ = peel_lanes; }
+  // Settings the batch autotuner varies (every value computes the same result).
+  struct Tuning {
+    int count_lanes = 0, peel_lanes = 0;  // 0: by average degree
+    bool edge_count = false;              // per-edge support kernel
+    double drop_ratio = 3.0;              // sparse peel: recount when frontier * ratio >= live edges
+  };
+  Tuning tuning() const { return Tuning{lanes_count_, lanes_peel_, force_edge_count_, sparse_drop_ratio_}; }
+  void set_tuning(const Tuning& t) {
+    lanes_count_ = t.count_lanes;
+    lanes_peel_ = t.peel_lanes;
+    force_edge_count_ = t.edge_count;
+    sparse_drop_ratio_ = t.drop_ratio;
+  }
+  void reset_drop_ratio() { sparse_drop_ratio_ = default_drop_ratio_; }
   void set_force_edge_count(bool v) { force_edge_count_ = v; }
   void set_force_atomic_build(bool v) { force_atomic_build_ = v; }
   void set_core_mode(int m) { core_mode_ = m; }  // 0 auto, 1 sparse, 2 dense
@@ -199,7 +213,7 @@ class Solver {
   DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_;
   DevBuf bk_state_, bk_sup_, bk_bits_, bk_del_;  // threshold-probe backups
   uint64_t eidm_budget_ = uint64_t(4) << 30;  // bytes for the edge-id matrix
-  double dense_tri_cost_ = 4.0, sparse_drop_ratio_ = 3.0;
+  double dense_tri_cost_ = 4.0, sparse_drop_ratio_ = 3.0, default_drop_ratio_ = 3.0;
   bool trace_ = false;  // KT_TRACE=1: per-round frontier sizes on stderr
   bool gallop_ = true;
   uint32_t count_kind_ = 0, count_words_ = 0;  // the last timed support count

@@ -48,6 +48,7 @@ Solver::Solver(int device) : device_(device) {
   if (const char* v = std::getenv("KT_TRACE")) trace_ = v[0] == '1';
   if (const char* v = std::getenv("KT_EIDM_BUDGET_MB")) eidm_budget_ = uint64_t(std::atoll(v)) << 20;
   if (const char* v = std::getenv("KT_WALK_LEVELS")) walk_levels_ = std::max(1, std::atoi(v));
+  default_drop_ratio_ = sparse_drop_ratio_;
 }
 
 template <typename F>

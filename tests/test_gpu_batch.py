@@ -85,11 +85,12 @@ def test_batch_flags_invalid_graphs(ctx):
     _check_against_plain(ctx, pairs[:int(offs[8])], offs[:9], ns[:8], res)
 
 
-def test_batch_repeat_and_no_autotune(ctx):
+def test_batch_repeat_and_autotune(ctx):
     """Repeated solves (with the compatibility graph flag, and with other solves
-    on the context in between) return the same answers; the autotuner entry
-    point reports that the union solve has nothing to tune."""
-    from kc_cuda_amd import KtError, gen
+    on the context in between) return the same answers; the autotuner's
+    trials agree, its final solve of the whole batch equals the untuned one,
+    and so does a solve with the settings it leaves on the context."""
+    from kc_cuda_amd import gen
     pairs, offs, ns = gen.batch(64, 32, 5000, 6.0, 30.0, 4, 24, 7)
     d = ctx.to_device(np.ascontiguousarray(pairs))
     try:
@@ -101,8 +102,22 @@ def test_batch_repeat_and_no_autotune(ctx):
             np.testing.assert_array_equal(r.edges, ref.edges)
             np.testing.assert_array_equal(r.keep, ref.keep)
             ctx.solve_pairs(pairs[:int(offs[1])], int(ns[0]))
-        with pytest.raises(KtError):
-            ctx.batch_autotune(d, offs, ns, per_bucket=4)
+        try:
+            for every in (3, 1):
+                r, t = ctx.batch_autotune(d, offs, ns, sample_every=every)
+                print(f"[gpu] batch autotune every={every}: {t}", flush=True)
+                assert 2 + 3 + 3 + 1 + 3 <= t["trials"] <= 2 + 4 + 4 + 1 + 3
+                assert t["peel_lanes"] in (0, 8, 16, 32, 64) and t["count_lanes"] in (0, 8, 16, 32, 64)
+                assert t["ms_best"] <= t["ms_default"]
+                np.testing.assert_array_equal(r.kmax, ref.kmax)
+                np.testing.assert_array_equal(r.edges, ref.edges)
+                np.testing.assert_array_equal(r.status, ref.status)
+                r2 = ctx.batch_solve(d, offs, ns, keep=True)
+                np.testing.assert_array_equal(r2.kmax, ref.kmax)
+                np.testing.assert_array_equal(r2.edges, ref.edges)
+                np.testing.assert_array_equal(r2.keep, ref.keep)
+        finally:
+            ctx.tune(0, 0, False)
     finally:
         d.free()
 
