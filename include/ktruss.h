@@ -93,7 +93,15 @@ int kt_solve_pairs_host(kt_context* ctx, const uint32_t* pairs, uint64_t m, uint
                         kt_result* res);
 int kt_solve_lines_host(kt_context* ctx, const uint32_t* xs, const uint32_t* ys, uint64_t L,
                         uint32_t n, kt_result* res);
+/* Edge-list text file (src/kc_gpu.cpp:243-312 parses it on the host): the
+ * bytes are streamed into HBM and parsed there (a file with a line holding a
+ * single number goes through the host parser instead, whose reading of such
+ * a line spans the next one). parse_ms: file to line arrays in HBM. */
 int kt_solve_tsv(kt_context* ctx, const char* path, kt_result* res, double* parse_ms);
+/* The device-side parse alone, lines copied back (at most cap of each);
+ * *on_device is 0 when the host parser had to take the file. */
+int kt_parse_tsv_device(kt_context* ctx, const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap,
+                        uint64_t* lines, uint32_t* n, int* on_device);
 
 /* Survivors of the last solve: original ids, u < v, unordered.
  * *count receives the truss size; at most cap pairs are written. */

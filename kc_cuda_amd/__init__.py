@@ -31,7 +31,7 @@ EXPORTS = [
     "kt_last_error", "kt_device_count", "kt_create", "kt_destroy", "kt_malloc", "kt_free",
     "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_memcpy_d2d", "kt_synchronize", "kt_solve_pairs_device",
     "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges",
-    "kt_parse_tsv", "kt_set_tuning", "kt_last_csr", "kt_last_stats", "kt_batch_solve",
+    "kt_parse_tsv", "kt_parse_tsv_device", "kt_set_tuning", "kt_last_csr", "kt_last_stats", "kt_batch_solve",
     "kt_batch_autotune", "kt_serve_create", "kt_serve_destroy", "kt_serve_submit", "kt_serve_wait",
     "kt_serve_counters_get", "kt_serve_footprint",
 ]
@@ -125,6 +125,8 @@ def native_library() -> ctypes.CDLL:
         "kt_set_tuning": [_V, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "kt_last_csr": [_V, _V, _V],
         "kt_last_stats": [_V, ctypes.POINTER(KtStats)],
+        "kt_parse_tsv_device": [_V, ctypes.c_char_p, _V, _V, _U64, ctypes.POINTER(_U64), ctypes.POINTER(_U32),
+                                ctypes.POINTER(ctypes.c_int)],
         "kt_batch_solve": [_V, _V, _V, _V, _U32, _V, _V, _V, _V, ctypes.c_int],
         "kt_batch_autotune": [_V, _V, _V, _V, _U32, _U32, _V, _V, _V, ctypes.POINTER(KtBatchTuning)],
         "kt_serve_create": [ctypes.c_int, ctypes.c_int, _U64, ctypes.POINTER(_V)],
@@ -248,6 +250,19 @@ class Context:
                                              ys.ctypes.data if ys.size else None, xs.size, n,
                                              ctypes.byref(r)))
         return r
+
+    def parse_tsv_device(self, path: str | os.PathLike) -> tuple[np.ndarray, np.ndarray, int, bool]:
+        """The device-side parse of an edge-list file: (xs, ys, n, on_device);
+        on_device is False when the host parser took the file."""
+        lines, n, dev = _U64(), _U32(), ctypes.c_int()
+        _check(self._lib.kt_parse_tsv_device(self._h, os.fsencode(str(path)), None, None, 0,
+                                             ctypes.byref(lines), ctypes.byref(n), ctypes.byref(dev)))
+        L = lines.value
+        xs = np.zeros(max(L, 1), dtype=np.uint32)
+        ys = np.zeros(max(L, 1), dtype=np.uint32)
+        _check(self._lib.kt_parse_tsv_device(self._h, os.fsencode(str(path)), xs.ctypes.data, ys.ctypes.data,
+                                             L, ctypes.byref(lines), ctypes.byref(n), ctypes.byref(dev)))
+        return xs[:L], ys[:L], n.value, bool(dev.value)
 
     def solve_tsv(self, path: str | os.PathLike) -> tuple[KtResult, float]:
         r = KtResult()

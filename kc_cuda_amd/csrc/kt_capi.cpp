@@ -186,10 +186,38 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
   });
 }
 
+int kt_parse_tsv_device(kt_context* ctx, const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap,
+                        uint64_t* lines, uint32_t* n, int* on_device) {
+  if (!ctx || !path) return set_err("kt_parse_tsv_device: NULL argument");
+  return guarded([&] {
+    uint64_t L = 0;
+    uint32_t nn = 0;
+    const bool dev = ctx->solver->ingest_tsv(path, &L, &nn);
+    if (on_device) *on_device = dev ? 1 : 0;
+    if (!dev) {  // a line with a single number: the host parser's reading
+      kt::ParsedLines p = kt::parse_tsv_file(path);
+      L = p.xs.size();
+      nn = p.n;
+      if (xs && ys) {
+        const uint64_t take = std::min<uint64_t>(cap, L);
+        std::memcpy(xs, p.xs.data(), sizeof(uint32_t) * take);
+        std::memcpy(ys, p.ys.data(), sizeof(uint32_t) * take);
+      }
+    } else if (xs && ys) {
+      ctx->solver->copy_ingested(xs, ys, std::min<uint64_t>(cap, L));
+    }
+    if (lines) *lines = L;
+    if (n) *n = nn;
+  });
+}
+
 int kt_solve_tsv(kt_context* ctx, const char* path, kt_result* res, double* parse_ms) {
   if (!ctx || !res || !path) return set_err("kt_solve_tsv: NULL argument");
+  bool done = false;
+  int rc = guarded([&] { done = ctx->solver->solve_tsv_device(path, res, parse_ms); });
+  if (rc || done) return rc;
   kt::ParsedLines p;
-  int rc = guarded([&] {
+  rc = guarded([&] {
     auto t0 = std::chrono::steady_clock::now();
     p = kt::parse_tsv_file(path);
     auto t1 = std::chrono::steady_clock::now();

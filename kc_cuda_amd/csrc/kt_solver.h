@@ -95,6 +95,13 @@ class Solver {
   // del_thr: an edge deleted at threshold t has truss number t + 1, an edge
   // never deleted has the returned kmax (0 when the 2-core has no edges).
   uint32_t decompose_pairs_device(const uint32_t* d_pairs, uint64_t m, uint32_t n, CoreView* cv);
+  // Text edge list (the reference's contract) read into HBM and parsed
+  // there, then solved; false (nothing solved) when a line holds a single
+  // number, which the host parser handles. parse_ms: file to line arrays.
+  bool solve_tsv_device(const char* path, kt_result* res, double* parse_ms);
+  // The parse alone; the lines stay on the device (copy_ingested reads them).
+  bool ingest_tsv(const char* path, uint64_t* lines, uint32_t* n);
+  void copy_ingested(uint32_t* xs, uint32_t* ys, uint64_t count);
   // Directed line list (x into row y) resident on the device.
   void solve_lines_device(const uint32_t* d_xs, const uint32_t* d_ys, uint64_t L, uint32_t n,
                           kt_result* res);
@@ -237,6 +244,10 @@ class Solver {
       in_src_, off_u_, adj_u_, eid_u_, sup_, state_, eq0_, eq1_, out_pairs_, del_thr_, orig0_;
   EdgeSet work_[2];
   DevBuf temp_;
+  // device-side text ingest: bytes, per-tile line counts and offsets, lines
+  DevBuf text_, tile_cnt_, tile_off_, lx_, ly_, lkind_, lx2_, ly2_, lkeep_, lkeep_off_, lstat_;
+  uint8_t* stage_[2] = {nullptr, nullptr};  // pinned staging of the file
+  hipEvent_t stage_ev_[2] = {nullptr, nullptr};
   CoreView last_core_{};
   int32_t last_floor_ = -1;
   uint64_t last_truss_ = 0;

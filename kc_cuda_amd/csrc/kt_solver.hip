@@ -41,6 +41,7 @@ Solver::Solver(int device) : device_(device) {
   KT_HIP(hipMemsetAsync(ctl_, 0, sizeof(Ctl), stream_));
   KT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 256, hipHostMallocDefault));
   for (auto& e : ev_) KT_HIP(hipEventCreate(&e));
+  for (auto& e : stage_ev_) KT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   KT_HIP(hipStreamSynchronize(stream_));
   // cost-model knobs of the frontier-versus-recount choice (tuning only)
   if (const char* v = std::getenv("KT_DENSE_TRI_COST")) dense_tri_cost_ = std::atof(v);
@@ -58,7 +59,9 @@ void Solver::each_buffer(F f) const {
                          &adj_d_, &src_, &din_, &din_off_, &tkeys_, &tvals_, &tiota_, &in_src_, &off_u_,
                          &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &del_thr_, &orig0_,
                          &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &fbits_, &pref0_, &eidm_, &heavy_,
-                         &vflag_, &vmap_, &roff_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_};
+                         &vflag_, &vmap_, &roff_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_, &text_,
+                         &tile_cnt_, &tile_off_, &lx_, &ly_, &lkind_, &lx2_, &ly2_, &lkeep_, &lkeep_off_,
+                         &lstat_};
   for (const DevBuf* b : all) f(*b);
   for (auto& w : work_)
     for (const DevBuf* b : {&w.off, &w.adj, &w.src, &w.orig, &w.sup}) f(*b);
@@ -83,6 +86,10 @@ Solver::~Solver() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   each_buffer([](const DevBuf& b) { const_cast<DevBuf&>(b).release(); });
   for (auto& e : ev_) (void)hipEventDestroy(e);
+  for (auto& e : stage_ev_)
+    if (e) (void)hipEventDestroy(e);
+  for (auto* p : stage_)
+    if (p) (void)hipHostFree(p);
   if (ctl_) (void)hipFree(ctl_);
   if (pinned_) (void)hipHostFree(pinned_);
   if (stream_) (void)hipStreamDestroy(stream_);
