@@ -107,6 +107,15 @@ int kt_parse_tsv_device(kt_context* ctx, const char* path, uint32_t* xs, uint32_
  * *count receives the truss size; at most cap pairs are written. */
 int kt_truss_edges(kt_context* ctx, uint32_t* pairs, uint64_t cap, uint64_t* count);
 
+/* Per-edge trussness of the last solved graph (no counterpart in the
+ * reference, which reports kmax and |T| only): a
+ * full truss decomposition on the device, then (u, v, t) triples, u < v in
+ * original ids, for every edge of trussness t >= 3 (every other edge has
+ * trussness 2), unordered. *count receives how many; at most cap triples
+ * are written. Afterwards kt_truss_edges has no survivors to report until
+ * the next solve, so call it first. */
+int kt_trussness(kt_context* ctx, uint32_t* triples, uint64_t cap, uint64_t* count);
+
 /* Host TSV parser (the reference's input contract, src/kc_gpu.cpp:243-312).
  * Pass NULL buffers to get the line count and n only. */
 int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uint64_t* lines,

@@ -30,7 +30,7 @@ __all__ = ["Context", "KtResult", "KtError", "Server", "ServeResult", "kmax_trus
 EXPORTS = [
     "kt_last_error", "kt_device_count", "kt_create", "kt_destroy", "kt_malloc", "kt_free",
     "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_memcpy_d2d", "kt_synchronize", "kt_solve_pairs_device",
-    "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges",
+    "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges", "kt_trussness",
     "kt_parse_tsv", "kt_parse_tsv_device", "kt_set_tuning", "kt_last_csr", "kt_last_stats", "kt_batch_solve",
     "kt_batch_autotune", "kt_serve_create", "kt_serve_destroy", "kt_serve_submit", "kt_serve_wait",
     "kt_serve_counters_get", "kt_serve_footprint",
@@ -127,6 +127,7 @@ def native_library() -> ctypes.CDLL:
         "kt_last_stats": [_V, ctypes.POINTER(KtStats)],
         "kt_parse_tsv_device": [_V, ctypes.c_char_p, _V, _V, _U64, ctypes.POINTER(_U64), ctypes.POINTER(_U32),
                                 ctypes.POINTER(ctypes.c_int)],
+        "kt_trussness": [_V, _V, _U64, ctypes.POINTER(_U64)],
         "kt_batch_solve": [_V, _V, _V, _V, _U32, _V, _V, _V, _V, ctypes.c_int],
         "kt_batch_autotune": [_V, _V, _V, _V, _U32, _U32, _V, _V, _V, ctypes.POINTER(KtBatchTuning)],
         "kt_serve_create": [ctypes.c_int, ctypes.c_int, _U64, ctypes.POINTER(_V)],
@@ -281,6 +282,17 @@ class Context:
         out = out[: cnt.value]
         order = np.lexsort((out[:, 1], out[:, 0]))
         return out[order]
+
+    def trussness(self) -> np.ndarray:
+        """(u, v, t) rows, u < v, sorted, for every edge of the last solved
+        graph with trussness t >= 3 (the others have 2). Replaces the last
+        solve's survivors: call truss_edges() first."""
+        cnt = _U64()
+        _check(self._lib.kt_trussness(self._h, None, 0, ctypes.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 3), dtype=np.uint32)
+        _check(self._lib.kt_trussness(self._h, out.ctypes.data, cnt.value, ctypes.byref(cnt)))
+        out = out[: cnt.value]
+        return out[np.lexsort((out[:, 1], out[:, 0]))]
 
     def stats(self) -> KtStats:
         """Byte-model terms W(C+), W(C+ minus T+) of the last solve."""

@@ -19,6 +19,9 @@ void batch_truss(const CoreView& cv, uint32_t kall, const uint32_t* vstart, uint
                  hipStream_t s);
 void batch_results(const uint32_t* gk, const unsigned long long* gcnt, const uint32_t* status, uint32_t count,
                    uint32_t* kmax, unsigned long long* edges, uint32_t* status_out, hipStream_t s);
+// (u, v, t) triples (u < v, input ids) of the core edges of a decomposition
+// with truss number t >= 3, unordered; count (starts at 0) gets how many.
+void truss_triples(const CoreView& cv, uint32_t kall, uint32_t* out, uint32_t* count, hipStream_t s);
 void fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s);
 
 }  // namespace kt

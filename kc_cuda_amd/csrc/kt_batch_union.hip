@@ -129,6 +129,23 @@ __global__ __launch_bounds__(kBlock) void k_batch_results(const uint32_t* __rest
   }
 }
 
+// (u, v, t), u < v input ids, for every core edge of truss number t >= 3
+__global__ __launch_bounds__(kBlock) void k_truss_triples(const uint32_t* __restrict__ src0,
+                                                          const uint32_t* __restrict__ dst0,
+                                                          const uint32_t* __restrict__ inv,
+                                                          const uint32_t* __restrict__ del_thr, uint32_t mc0,
+                                                          uint32_t kall, uint32_t* out, uint32_t* count) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < mc0; e += gridDim.x * blockDim.x) {
+    const uint32_t t = truss_of(del_thr, e, kall);
+    if (t < 3) continue;
+    const uint32_t a = inv[src0[e]], b = inv[dst0[e]];
+    const uint32_t at = atomicAdd(count, 1u);
+    out[3 * uint64_t(at)] = min(a, b);
+    out[3 * uint64_t(at) + 1] = max(a, b);
+    out[3 * uint64_t(at) + 2] = t;
+  }
+}
+
 __global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v) {
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
     p[i] = v;
@@ -159,6 +176,12 @@ void batch_truss(const CoreView& cv, uint32_t kall, const uint32_t* vstart, uint
 void batch_results(const uint32_t* gk, const unsigned long long* gcnt, const uint32_t* status, uint32_t count,
                    uint32_t* kmax, unsigned long long* edges, uint32_t* status_out, hipStream_t s) {
   if (count) k_batch_results<<<grid_for(count), kBlock, 0, s>>>(gk, gcnt, status, count, kmax, edges, status_out);
+}
+
+void truss_triples(const CoreView& cv, uint32_t kall, uint32_t* out, uint32_t* count, hipStream_t s) {
+  if (cv.mc0)
+    k_truss_triples<<<grid_for(cv.mc0), kBlock, 0, s>>>(cv.src0, cv.adj_d0, cv.inv, cv.del_thr, cv.mc0, kall, out,
+                                                        count);
 }
 
 void fill_u8(uint8_t* p, uint64_t n, uint8_t v, hipStream_t s) {

@@ -235,6 +235,14 @@ int kt_truss_edges(kt_context* ctx, uint32_t* pairs, uint64_t cap, uint64_t* cou
   });
 }
 
+int kt_trussness(kt_context* ctx, uint32_t* triples, uint64_t cap, uint64_t* count) {
+  if (!ctx) return set_err("kt_trussness: NULL context");
+  return guarded([&] {
+    const uint64_t c = ctx->solver->trussness(triples, cap);
+    if (count) *count = c;
+  });
+}
+
 int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
   if (!ctx) return set_err("kt_set_tuning: NULL context");
   auto ok = [](int l) { return l == 0 || l == 8 || l == 16 || l == 32 || l == 64; };

@@ -107,6 +107,11 @@ class Solver {
                           kt_result* res);
   // Copies the survivors of the last solve (original ids, u < v) to host.
   uint64_t truss_edges(uint32_t* host_pairs, uint64_t cap);
+  // Truss decomposition of the last input graph: (u, v, t) with u < v for
+  // every edge of trussness t >= 3 (the others have 2), unordered; returns
+  // the count, copies at most cap triples. Replaces the last solve's
+  // survivors (truss_edges fails afterwards).
+  uint64_t trussness(uint32_t* host_triples, uint64_t cap);
   // Byte-model terms of the last pass-2 core.
   void last_stats(kt_stats* out);
   // Stage access for parity tests: device CSR of the last solve.

@@ -20,6 +20,7 @@
 #include <climits>
 #include <cstring>
 
+#include "kt_batch_union.h"
 #include "kt_kernels.h"
 #include "kt_solver.h"
 
@@ -757,6 +758,33 @@ void Solver::last_stats(kt_stats* out) {
   tmp.release();
   out->w_core = h[0];
   out->w_peeled = h[1];
+}
+
+uint64_t Solver::trussness(uint32_t* host_triples, uint64_t cap) {
+  KT_HIP(hipSetDevice(device_));
+  if (!n_) return 0;
+  // full decomposition of the last input's CSR (level 0 is never modified
+  // after its build); the last solve's truss record is replaced
+  have_truss_ = false;
+  depth_ = 1;
+  LevelGraph& g = level0();
+  degrees_from_offsets(g.off, g.n, g.deg, stream_);
+  fill_u32(g.lvl, g.n, kNone, stream_);
+  peel_vertices(g, 2);  // a vertex of degree < 2 is in no triangle
+  CoreView cv;
+  extract_core(g, &cv);
+  if (!cv.mc0) return 0;
+  count_support(&cv, nullptr);
+  const uint32_t kall = truss_search(&cv, 1, 0, 0).kmax;
+  out_pairs_.ensure(sizeof(uint32_t) * 3 * size_t(cv.mc0));
+  zero_u32(&ctl_->nalive);
+  truss_triples(cv, kall, out_pairs_.as<uint32_t>(), &ctl_->nalive, stream_);
+  const uint32_t cnt = read_u32(&ctl_->nalive);
+  const uint64_t take = std::min<uint64_t>(cnt, cap);
+  if (take && host_triples)
+    KT_HIP(hipMemcpyAsync(host_triples, out_pairs_.p, sizeof(uint32_t) * 3 * take, hipMemcpyDeviceToHost, stream_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  return cnt;
 }
 
 uint64_t Solver::truss_edges(uint32_t* host_pairs, uint64_t cap) {

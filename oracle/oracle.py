@@ -207,6 +207,46 @@ def solve_pairs(pairs, n=None, want_edges: bool = False) -> Solve:
     return solve_csr(off, adj, n, want_edges)
 
 
+def trussness(pairs) -> dict:
+    """Per-edge trussness {(u, v): t}, u < v, by the textbook peel (Wang and
+    Cheng, "Truss decomposition in massive networks", PVLDB 2012): at level
+    k (from 2) remove, one at a time, the edges whose support among the
+    remaining edges is <= k - 2, updating their triangles' other edges; a
+    removed edge has trussness k. The reference reports only kmax and |T|
+    (src/kc_cpu.cpp:474-488 is its per-level delete rule): edges with
+    t = kmax are its kmax-truss. Pure-Python loops: small graphs only."""
+    adj: dict[int, set] = {}
+    edges = set()
+    for a, b in np.asarray(pairs, dtype=np.int64).reshape(-1, 2).tolist():
+        if a == b:
+            continue
+        u, v = (a, b) if a < b else (b, a)
+        edges.add((u, v))
+        adj.setdefault(u, set()).add(v)
+        adj.setdefault(v, set()).add(u)
+    sup = {e: len(adj[e[0]] & adj[e[1]]) for e in edges}
+    out = {}
+    k = 2
+    while sup:
+        queue = [e for e, s in sup.items() if s <= k - 2]
+        while queue:
+            e = queue.pop()
+            if e not in sup:
+                continue
+            u, v = e
+            for w in adj[u] & adj[v]:
+                for f in ((min(u, w), max(u, w)), (min(v, w), max(v, w))):
+                    sup[f] -= 1
+                    if sup[f] == k - 2:
+                        queue.append(f)
+            adj[u].discard(v)
+            adj[v].discard(u)
+            del sup[e]
+            out[e] = k
+        k += 1
+    return out
+
+
 # ------------------------------------------------ the reference's kc_cpu ---
 
 _RESULT = re.compile(rb"kmax = (\d+), Edges in kmax-truss = (\d+)\.")

@@ -233,3 +233,70 @@ def test_planted_support_error_is_caught():
     r = int(np.searchsorted(off_d, j, side="right") - 1)
     u, v = int(inv[r]), int(inv[adj_d[j]])
     assert bad[j] != 2 * len(nb[u] & nb[v])
+
+
+def _truss_by_definition(edges):
+    """t(e) = the largest k whose k-truss holds e, each k-truss found as the
+    fixpoint of deleting edges with fewer than k - 2 triangles (supports
+    recounted from scratch every pass)."""
+    out = {e: 2 for e in edges}
+    k = 3
+    cur = set(edges)
+    while cur:
+        while True:
+            adj = {}
+            for u, v in cur:
+                adj.setdefault(u, set()).add(v)
+                adj.setdefault(v, set()).add(u)
+            drop = {(u, v) for u, v in cur if len(adj[u] & adj[v]) < k - 2}
+            if not drop:
+                break
+            cur -= drop
+        for e in cur:
+            out[e] = k
+        k += 1
+    return out
+
+
+def test_trussness_matches_definition_on_small_graphs():
+    import itertools
+    rng = np.random.default_rng(11)
+    for trial in range(40):
+        n = int(rng.integers(4, 13))
+        p = float(rng.uniform(0.2, 0.9))
+        edges = [(a, b) for a, b in itertools.combinations(range(n), 2) if rng.random() < p]
+        if not edges:
+            continue
+        got = O.trussness(np.array(edges))
+        assert got == _truss_by_definition(edges), trial
+
+
+def test_trussness_closed_forms():
+    import itertools
+    def clique(vs):
+        return [(a, b) for a, b in itertools.combinations(vs, 2)]
+    k5_tail = clique(range(5)) + [(4, 5), (5, 6)]
+    t = O.trussness(np.array(k5_tail))
+    assert all(t[e] == 5 for e in clique(range(5))) and t[(4, 5)] == 2 and t[(5, 6)] == 2
+    parts = clique(range(4)) + clique(range(10, 16)) + clique(range(20, 23))
+    t = O.trussness(np.array(parts))
+    assert all(t[e] == 4 for e in clique(range(4)))
+    assert all(t[e] == 6 for e in clique(range(10, 16)))
+    assert all(t[e] == 3 for e in clique(range(20, 23)))
+    cycle = [(i, (i + 1) % 9) for i in range(9)]
+    assert set(O.trussness(np.array(cycle)).values()) == {2}
+
+
+def test_trussness_top_level_is_the_kmax_truss():
+    """The edges of the largest trussness are the oracle's kmax-truss (pinned
+    against kc_cpu and networkx elsewhere), on random graphs with cliques."""
+    for seed in range(6):
+        pairs, n = _graph(seed, 300)
+        t = O.trussness(pairs)
+        s = O.solve_pairs(pairs, n, want_edges=True)
+        top = max(t.values())
+        kmax = top if top >= 3 else 0
+        assert kmax == s.kmax
+        got = sorted(e for e, k in t.items() if k == top) if kmax else []
+        want = sorted(map(tuple, np.sort(s.truss, axis=1).tolist())) if kmax else []
+        assert got == want
