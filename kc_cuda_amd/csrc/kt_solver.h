@@ -169,7 +169,8 @@ class Solver {
                             bool exact_early = false);
   // Dense views only: the same answer as truss_search(cv, 1, 0, 0) found by
   // probing thresholds with backups instead of walking every level.
-  TrussOutcome truss_gallop(CoreView* cv);
+  // empty_from: a threshold known to leave the view empty (-1: none)
+  TrussOutcome truss_gallop(CoreView* cv, int64_t empty_from);
 
   // reference-compatible stage entry points (kt_compat.hip)
   void vertex_state_from_host(const uint32_t* deg_origin, const uint32_t* mask);

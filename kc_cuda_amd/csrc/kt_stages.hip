@@ -541,13 +541,15 @@ TrussOutcome Solver::run_levels(CoreView* cv, int32_t thr, uint64_t alive, uint3
 // not over by then, thresholds are probed instead: each probe peels one
 // level at t from the last non-empty state (kept in a backup) and is undone
 // when it empties the view. Thresholds gallop up by doubling steps from the
-// minimum support, then bisect until at most kWalkWindow thresholds are left,
-// whose levels are then walked. The result equals the level walk's: the last
+// minimum support (never past empty_from - 1, a bound known to hold, so a
+// view whose answer sits at the bound is done by one kept probe), then
+// bisect until at most kWalkWindow thresholds are left, whose levels are
+// then walked. The result equals the level walk's: the last
 // non-empty probe leaves the t-level, whose minimum support is t* (a level at
 // t* + 1 is empty), so kmax = t* + 2 with its edge count, and every edge
 // deleted on the way carries a threshold <= the floor. (The reference's
 // speculative k-steps with backups, src/kc.cu:1189-1196, 1252-1272.)
-TrussOutcome Solver::truss_gallop(CoreView* cv) {
+TrussOutcome Solver::truss_gallop(CoreView* cv, int64_t empty_from) {
   constexpr int64_t kWalkWindow = 32;  // thresholds left when probing gives way to walking
   TrussOutcome out;
   if (cv->mc == 0) return out;
@@ -585,11 +587,12 @@ TrussOutcome Solver::truss_gallop(CoreView* cv) {
   SearchRule single{};
   single.max_levels = 1;
   int64_t lo = int64_t(out.kmax) - 2;  // non-empty: the current state is its level
-  int64_t hi = -1;                      // smallest known empty threshold
+  int64_t hi = empty_from;              // smallest threshold known to be empty (-1: none)
+  bool probed_empty = false;            // hi comes from a probe, not from the bound
   uint64_t alive = out.count;
   int64_t step = 1;
   while (hi < 0 || hi - lo > 1) {
-    if (hi >= 0 && hi - lo <= kWalkWindow) {
+    if (probed_empty && hi - lo <= kWalkWindow) {
       // A short window is left: walk its levels. A probe that empties the
       // view costs a full cascade and is then undone; the walk's only empty
       // level is the last one.
@@ -608,7 +611,9 @@ TrussOutcome Solver::truss_gallop(CoreView* cv) {
                      (unsigned long long)out.count);
       break;
     }
-    const int64_t t = hi < 0 ? lo + step : lo + (hi - lo) / 2;
+    // gallop up by doubling steps, never past the bound; bisect once a
+    // probe has emptied the view
+    const int64_t t = hi < 0 ? lo + step : !probed_empty ? std::min(lo + step, hi - 1) : lo + (hi - lo) / 2;
     save();
     const TrussOutcome p = run_levels(cv, int32_t(t), alive, &r, single, false);
     out.rounds += p.rounds;
@@ -624,6 +629,7 @@ TrussOutcome Solver::truss_gallop(CoreView* cv) {
     } else {
       restore();
       hi = t;
+      probed_empty = true;
     }
     if (trace_)
       std::fprintf(stderr, "[kt] gallop probe t=%lld -> %s lo=%lld hi=%lld alive=%llu\n", (long long)t,
@@ -696,7 +702,10 @@ void Solver::solve_csr(kt_result* res) {
   CoreView cv;
   extract_core(*levels_[depth_ - 1], &cv);
   count_support(&cv, nullptr);
-  TrussOutcome p1 = cv.dense && gallop_ ? truss_gallop(&cv) : truss_search(&cv, 1, 0, 0);
+  // the max-core's kmax is at most max_core + 1 (a k-truss is a
+  // (k-1)-core), so its threshold max_core is empty
+  TrussOutcome p1 =
+      cv.dense && gallop_ ? truss_gallop(&cv, int64_t(res->max_core)) : truss_search(&cv, 1, 0, 0);
   res->k_hat = p1.kmax;
   res->k_hat_edges = p1.count;
   KT_HIP(hipEventRecord(ev_[3], stream_));
