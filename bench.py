@@ -29,13 +29,18 @@ Poisson run, with p50/p99 latency (submit to answer).
 
 Prints one JSON line on stdout (everything else goes to stderr):
   value        edges/s = m / wall time per solve
-  roofline     the pass-2 support-count kernel (the count the reference
-               spends most of its device time on): algorithmic bytes
-               4*W(C+) + 4*m_C + 4*(n_C+1) per launch (one exact merge over
-               every oriented row pair, support write, offsets) over its
-               HIP-event duration, against 8 TB/s HBM3E; `traffic` is the
-               rocprofv3-measured HBM bytes of the same kernel when a summary
-               for this workload is committed under profiles/
+  roofline     the largest kernel of the solve, the vertex-peel push
+               (k_core_push): algorithmic bytes per launch (per row slot
+               walked 12 B: adj, lvl, degree atomic; per vertex taken 20 B:
+               queue entry, two offsets) over its average HIP-event launch
+               duration, from one extra untimed solve with every push
+               launch bracketed by events on the solver's stream, against
+               8 TB/s HBM3E; `traffic` is the rocprofv3-measured HBM bytes
+               per launch when a summary for this workload is committed
+               under profiles/. config.count_kernel_roofline gives the same
+               for the pass-2 support count (the count the reference spends
+               most of its device time on; merge-model bytes 4*W(C+) +
+               4*m_C + 4*(n_C+1))
   cpu_baseline the reference's kc_cpu (compiled from its sources into
                oracle/_ref) on the same graph written as TSV, timed like the
                reference's own "All time cost - Reading file Time Cost";
@@ -173,13 +178,26 @@ def main() -> int:
     count_bytes = 4 * st.w_core + 4 * st.m_core + 4 * (st.n_core + 1)
     count_s = stages["ms_count_kernel"] / 1e3
     achieved = count_bytes / count_s / 1e9 if count_s > 0 else 0.0
-    traffic = None
-    tfile = PROFILES / f"{kernel}_{args.workload}_traffic.json"
-    if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
+    traffic = _traffic(kernel, args.workload)
+    # the largest kernel of a solve, the vertex-peel push: one more solve
+    # (untimed) with HIP events stamped at every push launch's start and end
+    # on the solver's stream, and one that counts what the launches walk;
+    # algorithmic bytes: per row slot walked 4 B adj + 4 B lvl + 4 B degree
+    # atomic, per vertex taken 4 B queue + 16 B offsets
+    ctx.push_probe(1)  # time: events only
+    ctx.solve_device(d, m, n)
+    ps = ctx.push_stats()
+    ctx.push_probe(2)  # bytes: slot and vertex counters (their atomics stay out of the timing)
+    ctx.solve_device(d, m, n)
+    counted = ctx.push_stats()
+    ctx.push_probe(0)
+    ps["slots"], ps["vertices"] = counted["slots"], counted["vertices"]
+    push_launches = max(int(ps["launches"]), 1)
+    push_bytes = (12 * ps["slots"] + 20 * ps["vertices"]) / push_launches
+    push_us = ps["ms"] * 1e3 / push_launches
+    push_achieved = push_bytes / (push_us * 1e-6) / 1e9 if push_us > 0 else 0.0
+    log(f"[bench] k_core_push: {ps['launches']} launches, {push_us:.1f} us avg, "
+        f"{ps['slots']} slots, {ps['vertices']} vertices -> {push_achieved:.1f} GB/s")
     if args.calibrate_copy_mib > 0:
         nb = args.calibrate_copy_mib << 20
         a, b = ctx.alloc(nb), ctx.alloc(nb)
@@ -220,14 +238,32 @@ def main() -> int:
                    "n_core": st.n_core, "m_core": int(st.m_core), "w_core": int(st.w_core),
                    "w_peeled": int(st.w_peeled), "stages_ms": stages,
                    "solve_bytes_model": model["total"],
-                   "solve_roofline_frac": model["total"] / (wall * HBM_PEAK_GBS * 1e9)},
-        "roofline": {"kernel": kernel, "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": count_bytes},
+                   "solve_roofline_frac": model["total"] / (wall * HBM_PEAK_GBS * 1e9),
+                   "count_kernel_roofline": {
+                       "kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                       "algorithmic_bytes": count_bytes}},
+        "roofline": {"kernel": "k_core_push", "bound": "hbm", "achieved": push_achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": push_achieved / HBM_PEAK_GBS,
+                     "traffic": _traffic("k_core_push", args.workload),
+                     "algorithmic_bytes": push_bytes, "launches": int(ps["launches"]),
+                     "avg_launch_us": push_us},
         "cpu_baseline": base,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _traffic(kernel: str, workload: str):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    tfile = PROFILES / f"{kernel}_{workload}_traffic.json"
+    if not tfile.exists():
+        return None
+    try:
+        return json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def _pct(xs, q):

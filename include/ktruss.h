@@ -224,6 +224,25 @@ int kt_serve_counters_get(kt_server* srv, kt_serve_counters* out);
 /* Admission charge of a query of m edges on n vertices (bytes). */
 uint64_t kt_serve_footprint(uint64_t m, uint32_t n);
 
+/* Measurement hook for the vertex-peel push kernel (k_core_push), the
+ * largest kernel of a solve. mode bit 1 (KT_PROBE_TIME): each launch stamps
+ * HIP events at its own start and end on the context's stream; bit 2
+ * (KT_PROBE_COUNT): each launch counts the row slots it walks (rows up to
+ * 8192 slots; longer ones go to the sliced kernel) and the vertices it takes,
+ * with atomics that slow it, so time and count in separate solves. 0 turns
+ * the probe off. kt_push_stats_get returns the sums since the last
+ * kt_push_probe. */
+#define KT_PROBE_TIME 1
+#define KT_PROBE_COUNT 2
+typedef struct kt_push_stats {
+  uint64_t launches;
+  uint64_t slots;
+  uint64_t vertices;
+  double ms;  /* summed launch durations */
+} kt_push_stats;
+int kt_push_probe(kt_context* ctx, int mode);
+int kt_push_stats_get(kt_context* ctx, kt_push_stats* out);
+
 /* Byte-model terms of the last solve (one extra device pass, untimed). */
 int kt_last_stats(kt_context* ctx, kt_stats* out);
 

@@ -31,6 +31,7 @@ EXPORTS = [
     "kt_last_error", "kt_device_count", "kt_create", "kt_destroy", "kt_malloc", "kt_free",
     "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_memcpy_d2d", "kt_synchronize", "kt_solve_pairs_device",
     "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges", "kt_trussness",
+    "kt_push_probe", "kt_push_stats_get",
     "kt_parse_tsv", "kt_parse_tsv_device", "kt_set_tuning", "kt_last_csr", "kt_last_stats", "kt_batch_solve",
     "kt_batch_autotune", "kt_serve_create", "kt_serve_destroy", "kt_serve_submit", "kt_serve_wait",
     "kt_serve_counters_get", "kt_serve_footprint",
@@ -84,6 +85,11 @@ class KtBatchTuning(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class KtPushStats(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_uint64), ("slots", ctypes.c_uint64), ("vertices", ctypes.c_uint64),
+                ("ms", ctypes.c_double)]
+
+
 class KtServeCounters(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint64) for name in (
         "submitted", "completed", "rejected", "failed", "in_flight", "peak_in_flight",
@@ -128,6 +134,8 @@ def native_library() -> ctypes.CDLL:
         "kt_parse_tsv_device": [_V, ctypes.c_char_p, _V, _V, _U64, ctypes.POINTER(_U64), ctypes.POINTER(_U32),
                                 ctypes.POINTER(ctypes.c_int)],
         "kt_trussness": [_V, _V, _U64, ctypes.POINTER(_U64)],
+        "kt_push_probe": [_V, ctypes.c_int],
+        "kt_push_stats_get": [_V, ctypes.POINTER(KtPushStats)],
         "kt_batch_solve": [_V, _V, _V, _V, _U32, _V, _V, _V, _V, ctypes.c_int],
         "kt_batch_autotune": [_V, _V, _V, _V, _U32, _U32, _V, _V, _V, ctypes.POINTER(KtBatchTuning)],
         "kt_serve_create": [ctypes.c_int, ctypes.c_int, _U64, ctypes.POINTER(_V)],
@@ -282,6 +290,16 @@ class Context:
         out = out[: cnt.value]
         order = np.lexsort((out[:, 1], out[:, 0]))
         return out[order]
+
+    def push_probe(self, mode: int) -> None:
+        """Measurement of the vertex-peel push kernel: 1 stamps events at
+        each launch's start and end, 2 counts its slots and vertices, 0 off."""
+        _check(self._lib.kt_push_probe(self._h, int(mode)))
+
+    def push_stats(self) -> dict:
+        s = KtPushStats()
+        _check(self._lib.kt_push_stats_get(self._h, ctypes.byref(s)))
+        return {name: getattr(s, name) for name, _ in s._fields_}
 
     def trussness(self) -> np.ndarray:
         """(u, v, t) rows, u < v, sorted, for every edge of the last solved

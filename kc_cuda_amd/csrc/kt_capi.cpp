@@ -235,6 +235,16 @@ int kt_truss_edges(kt_context* ctx, uint32_t* pairs, uint64_t cap, uint64_t* cou
   });
 }
 
+int kt_push_probe(kt_context* ctx, int on) {
+  if (!ctx) return set_err("kt_push_probe: NULL context");
+  return guarded([&] { ctx->solver->set_push_probe(on); });
+}
+
+int kt_push_stats_get(kt_context* ctx, kt_push_stats* out) {
+  if (!ctx || !out) return set_err("kt_push_stats_get: NULL argument");
+  return guarded([&] { ctx->solver->push_stats(&out->launches, &out->ms, &out->slots, &out->vertices); });
+}
+
 int kt_trussness(kt_context* ctx, uint32_t* triples, uint64_t cap, uint64_t* count) {
   if (!ctx) return set_err("kt_trussness: NULL context");
   return guarded([&] {

@@ -40,10 +40,13 @@ void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, uint32_t B, uin
 // alive vertex of degree < k, then takes one degree off every neighbour of
 // the queued vertices. ctl->removed counts the removed vertices;
 // ctl->action turns idle after a round that found nothing to remove.
-// q holds n entries.
+// q holds n entries. Measurement (all nullable): push_stats[0] / [1] gain
+// the light row slots the push kernel walks and the vertices it takes; ev0
+// and ev1 (both or neither) are stamped at its start and end.
 void core_round_start(Ctl* ctl, hipStream_t s);
 void core_round(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t* q, Ctl* ctl, uint32_t k,
-                uint32_t* lvl, uint32_t* deg, HeavyList hl, hipStream_t s);
+                uint32_t* lvl, uint32_t* deg, HeavyList hl, unsigned long long* push_stats, hipEvent_t ev0,
+                hipEvent_t ev1, hipStream_t s);
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl,
                         uint32_t n, uint32_t* deg, HeavyList hl, hipStream_t s);
 // lvl[v] = (lvl[v] > k) ? kNone : (lvl[v] == kNone ? kNone : k) -- used to

@@ -10,6 +10,7 @@
 // The algorithms are restated for wave64 and LDS, not translated: frontier
 // queues replace full-array mask scans, hub rows live in LDS as a bitmap plus
 // a sorted list, and same-row credits are accumulated in LDS.
+#include <hip/hip_ext.h>
 #include "kt_device.h"
 #include "kt_kernels.h"
 
@@ -351,7 +352,8 @@ __global__ __launch_bounds__(kBlock) void k_core_push(const uint64_t* __restrict
                                                       const uint32_t* __restrict__ adj,
                                                       const uint32_t* __restrict__ q, const Ctl* ctl,
                                                       const uint32_t* __restrict__ lvl, uint32_t* deg,
-                                                      uint32_t* heavy, uint32_t* heavy_count) {
+                                                      uint32_t* heavy, uint32_t* heavy_count,
+                                                      unsigned long long* stats) {
   if (ctl->action != kRoundPeel) return;
   const uint32_t cnt = ctl->qcount[0];
   __shared__ uint64_t s_b[kBlock];
@@ -383,6 +385,10 @@ __global__ __launch_bounds__(kBlock) void k_core_push(const uint64_t* __restrict
       if (lane >= o) inc += y;
     }
     const uint32_t total = __shfl(inc, kWave - 1);
+    if (stats && part == 0 && lane == 0) {  // measurement: light slots walked, vertices taken
+      atomicAdd(&stats[0], (unsigned long long)total);
+      atomicAdd(&stats[1], (unsigned long long)min(uint32_t(kWave), cnt - grp * kWave));
+    }
     s_b[threadIdx.x] = b;
     s_inc[threadIdx.x] = inc;
     wave_sync();
@@ -1205,13 +1211,29 @@ void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStr
 
 void core_round_start(Ctl* ctl, hipStream_t s) { k_vround_start<<<1, kWave, 0, s>>>(ctl); }
 void core_round(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t* q, Ctl* ctl, uint32_t k,
-                uint32_t* lvl, uint32_t* deg, HeavyList hl, hipStream_t s) {
+                uint32_t* lvl, uint32_t* deg, HeavyList hl, unsigned long long* push_stats, hipEvent_t ev0,
+                hipEvent_t ev1, hipStream_t s) {
   constexpr int kPushGrid = 2048;
   if (!n) return;
   // grids sized by the vertex count: small graphs launch a few workgroups
   const int push_grid = int(std::min<uint64_t>(kPushGrid, std::max<uint64_t>(4, (uint64_t(n) + 63) / 64)));
   k_core_sweep<<<grid_tiles(n), kBlock, 0, s>>>(lvl, deg, n, k, q, ctl);
-  k_core_push<<<push_grid, kBlock, 0, s>>>(off, adj, q, ctl, lvl, deg, hl.items, hl.count);
+  if (ev0 && ev1) {  // measurement: events stamped at the kernel's own start and end
+    const uint64_t* a_off = off;
+    const uint32_t* a_adj = adj;
+    const uint32_t* a_q = q;
+    const Ctl* a_ctl = ctl;
+    const uint32_t* a_lvl = lvl;
+    uint32_t* a_deg = deg;
+    uint32_t* a_items = hl.items;
+    uint32_t* a_count = hl.count;
+    unsigned long long* a_stats = push_stats;
+    void* args[] = {&a_off, &a_adj, &a_q, &a_ctl, &a_lvl, &a_deg, &a_items, &a_count, &a_stats};
+    KT_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(&k_core_push), dim3(push_grid), dim3(kBlock), args, 0,
+                              s, ev0, ev1, 0));
+  } else {
+    k_core_push<<<push_grid, kBlock, 0, s>>>(off, adj, q, ctl, lvl, deg, hl.items, hl.count, push_stats);
+  }
   if (uint64_t(n) > kHeavyRow + 1) {  // else no row can be heavy
     k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices, ctl);
     k_core_push_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, hl.items, hl.count, hl.slices, lvl, deg, ctl);

@@ -117,6 +117,12 @@ class Solver {
   // Stage access for parity tests: device CSR of the last solve.
   void csr_to_host(uint64_t* off, uint32_t* adj);
   void set_lanes(int count_lanes, int peel_lanes) { lanes_count_ = count_lanes; lanes_peel_ = peel_lanes; }
+  // Measurement of the vertex-peel push kernel: mode bit 1 stamps events at
+  // each launch's start and end, bit 2 counts the light row slots it walks
+  // and the vertices it takes (separate solves keep the counters' atomics
+  // out of the timing); push_stats sums them since set_push_probe.
+  void set_push_probe(int mode);
+  void push_stats(uint64_t* launches, double* ms, uint64_t* slots, uint64_t* vertices);
   // Settings the batch autotuner varies (every value computes the same result).
   struct Tuning {
     int count_lanes = 0, peel_lanes = 0;  // 0: by average degree
@@ -231,6 +237,10 @@ class Solver {
   bool gallop_ = true;
   uint32_t count_kind_ = 0, count_words_ = 0;  // the last timed support count
   uint32_t walk_levels_ = 16;
+  int probe_push_ = 0;               // measurement of the push kernel (mode bits)
+  std::vector<hipEvent_t> probe_ev_;
+  size_t probe_used_ = 0;
+  DevBuf probe_stats_;
   void make_dense(CoreView* cv);
   HeavyList heavy() {
     heavy_.ensure(sizeof(uint32_t) * 2 * (size_t(n_) + 2));

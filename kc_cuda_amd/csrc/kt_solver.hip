@@ -61,7 +61,7 @@ void Solver::each_buffer(F f) const {
                          &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &fbits_, &pref0_, &eidm_, &heavy_,
                          &vflag_, &vmap_, &roff_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_, &text_,
                          &tile_cnt_, &tile_off_, &lx_, &ly_, &lkind_, &lx2_, &ly2_, &lkeep_, &lkeep_off_,
-                         &lstat_};
+                         &lstat_, &probe_stats_};
   for (const DevBuf* b : all) f(*b);
   for (auto& w : work_)
     for (const DevBuf* b : {&w.off, &w.adj, &w.src, &w.orig, &w.sup}) f(*b);
@@ -88,6 +88,7 @@ Solver::~Solver() {
   for (auto& e : ev_) (void)hipEventDestroy(e);
   for (auto& e : stage_ev_)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : probe_ev_) (void)hipEventDestroy(e);
   for (auto* p : stage_)
     if (p) (void)hipHostFree(p);
   if (ctl_) (void)hipFree(ctl_);
@@ -253,6 +254,34 @@ LevelGraph& Solver::level0() {
   g.k_enter = 0;
   g.ok.clear();
   return g;
+}
+
+void Solver::set_push_probe(int mode) {
+  KT_HIP(hipSetDevice(device_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  probe_push_ = mode & 3;
+  probe_used_ = 0;
+  if (probe_push_) {
+    probe_stats_.ensure(2 * sizeof(unsigned long long));
+    fill_u64(probe_stats_.as<uint64_t>(), 2, 0, stream_);
+  }
+}
+
+void Solver::push_stats(uint64_t* launches, double* ms, uint64_t* slots, uint64_t* vertices) {
+  KT_HIP(hipSetDevice(device_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  double total = 0;
+  for (size_t i = 0; i < probe_used_; ++i) {
+    float t = 0;
+    KT_HIP(hipEventElapsedTime(&t, probe_ev_[2 * i], probe_ev_[2 * i + 1]));
+    total += t;
+  }
+  uint64_t h[2] = {0, 0};
+  if (probe_stats_.p) KT_HIP(hipMemcpy(h, probe_stats_.p, sizeof(h), hipMemcpyDeviceToHost));
+  *launches = probe_used_;
+  *ms = total;
+  *slots = h[0];
+  *vertices = h[1];
 }
 
 }  // namespace kt

@@ -52,7 +52,22 @@ uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
   int batch = 2;
   uint64_t queued = 0;  // every round but the last removes a vertex
   for (;;) {
-    for (int i = 0; i < batch; ++i) core_round(g.off, g.adj, g.n, q, ctl_, k, g.lvl, g.deg, hl, stream_);
+    for (int i = 0; i < batch; ++i) {
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      unsigned long long* ps = nullptr;
+      if (probe_push_ & 1) {  // measurement: events at every push launch's start and end
+        if (2 * probe_used_ + 2 > probe_ev_.size())
+          for (int e = 0; e < 64; ++e) {
+            probe_ev_.push_back(nullptr);
+            KT_HIP(hipEventCreate(&probe_ev_.back()));
+          }
+        e0 = probe_ev_[2 * probe_used_];
+        e1 = probe_ev_[2 * probe_used_ + 1];
+        ++probe_used_;
+      }
+      if (probe_push_ & 2) ps = probe_stats_.as<unsigned long long>();  // measurement: slot counters
+      core_round(g.off, g.adj, g.n, q, ctl_, k, g.lvl, g.deg, hl, ps, e0, e1, stream_);
+    }
     queued += batch;
     const Ctl h = read_ctl();
     if (trace_)

@@ -192,3 +192,28 @@ def test_bad_vertex_id_raises(ctx):
     pairs = np.array([[0, 1], [1, 7]], dtype=np.uint32)
     with pytest.raises(KtError):
         ctx.solve_pairs(pairs, 5)
+
+
+def test_push_probe_keeps_results(ctx):
+    """The measurement hook of the vertex-peel push (events at each launch's
+    start and end, slot counters) leaves every answer unchanged."""
+    from kc_cuda_amd import gen
+    pairs, n = gen.rmat(15, 16, 0.57, 0.19, 0.19, True, 16, 8)
+    ref = ctx.solve_pairs(pairs, n)
+    t_ref = ctx.truss_edges()
+    try:
+        ctx.push_probe(1)
+        r = ctx.solve_pairs(pairs, n)
+        timed = ctx.push_stats()
+        np.testing.assert_array_equal(ctx.truss_edges(), t_ref)
+        ctx.push_probe(3)
+        r2 = ctx.solve_pairs(pairs, n)
+        both = ctx.push_stats()
+    finally:
+        ctx.push_probe(0)
+    for x in (r, r2):
+        assert (x.kmax, x.truss_edges, x.max_core, x.k_hat) == (ref.kmax, ref.truss_edges, ref.max_core, ref.k_hat)
+    np.testing.assert_array_equal(ctx.truss_edges(), t_ref)
+    assert timed["launches"] > 0 and timed["ms"] > 0 and timed["slots"] == 0
+    assert both["launches"] == timed["launches"] and 0 < both["slots"] <= 2 * len(pairs) * 4
+    assert 0 < both["vertices"]
