@@ -120,7 +120,6 @@ void Server::finish(uint32_t s, std::vector<Query>& group, std::vector<ServeOutc
     ctr_.bytes_in_use = ctr_.bytes_in_use - slot.charge + slot.held;
     slot.charge = slot.held;
     ctr_.peak_bytes_in_use = std::max(ctr_.peak_bytes_in_use, ctr_.bytes_in_use);
-    slot.busy = false;
     busy_ -= 1;
   }
   cv_admit_.notify_all();  // memory and a slot freed up
@@ -257,7 +256,6 @@ void Server::run_slot(uint32_t s) {
       } else {
         group.push_back(queue_.front());
         queue_.pop_front();
-        slot.busy = true;
         busy_ += 1;
         // its buffers hold slot.held now and grow by at most the queries' footprints
         const uint64_t charge = slot.held + group[0].bytes;

@@ -104,7 +104,6 @@ class Server {
     DevBuf gres;         // a group's per-query kmax, |T| and status
     uint64_t held = 0;   // device bytes its buffers hold (set by its own thread)
     uint64_t charge = 0; // bytes counted in bytes_in_use for it
-    bool busy = false;   // solving (guarded by mu_)
     std::thread thread;
   };
   uint64_t slot_bytes(Slot& s) const {
