@@ -102,6 +102,9 @@ def test_batch_repeat_and_autotune(ctx):
             np.testing.assert_array_equal(r.edges, ref.edges)
             np.testing.assert_array_equal(r.keep, ref.keep)
             ctx.solve_pairs(pairs[:int(offs[1])], int(ns[0]))
+        from kc_cuda_amd import KtError
+        with pytest.raises(KtError, match="sample_every"):
+            ctx.batch_autotune(d, offs, ns, sample_every=0)
         try:
             for every in (3, 1):
                 r, t = ctx.batch_autotune(d, offs, ns, sample_every=every)
