@@ -15,8 +15,10 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <string>
+#include <thread>
 
 #include "kt_device.h"
 #include "kt_kernels.h"
@@ -39,16 +41,36 @@ struct Fd {
   }
 };
 
-// Line starts in [c0, c1): q == 0 or byte q - 1 is '\n' (q < len).
-__device__ __forceinline__ uint32_t count_starts(const uint8_t* __restrict__ text, uint64_t len, uint64_t c0) {
-  const uint64_t c1 = min(c0 + kChunkBytes, len);
-  uint32_t c = 0;
-  uint8_t prev = c0 ? text[c0 - 1] : uint8_t('\n');
-  for (uint64_t q = c0; q < c1; ++q) {
-    c += prev == '\n';
-    prev = text[q];
+// A thread's 64-byte chunk [c0, c0 + 64) in registers (text is allocated in
+// whole tiles, so the load stays inside the buffer; bytes at or past len are
+// ignored).
+struct Chunk64 {
+  uint32_t w[16];
+  __device__ __forceinline__ void load(const uint8_t* __restrict__ text, uint64_t c0) {
+    const uint4* p = reinterpret_cast<const uint4*>(text + c0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint4 v = p[k];
+      w[4 * k] = v.x;
+      w[4 * k + 1] = v.y;
+      w[4 * k + 2] = v.z;
+      w[4 * k + 3] = v.w;
+    }
   }
+  __device__ __forceinline__ uint8_t byte(int i) const { return uint8_t(w[i >> 2] >> (8 * (i & 3))); }
+};
+
+// Line starts in the chunk: q == 0 or byte q - 1 is '\n', for c0 <= q < c0 + nb
+// (nb = bytes of the chunk before len); first: whether c0 itself starts one.
+__device__ __forceinline__ uint32_t count_starts(const Chunk64& ch, uint32_t nb, bool first) {
+  uint32_t c = first ? 1u : 0u;
+#pragma unroll
+  for (int i = 0; i < int(kChunkBytes) - 1; ++i) c += (uint32_t(i) + 1 < nb) && ch.byte(i) == '\n';
   return c;
+}
+
+__device__ __forceinline__ bool starts_at(const uint8_t* __restrict__ text, uint64_t c0) {
+  return c0 == 0 || text[c0 - 1] == '\n';
 }
 
 // lines per tile (the workgroup pass of kTileBytes bytes)
@@ -56,75 +78,111 @@ __global__ __launch_bounds__(kBlock) void k_tsv_count(const uint8_t* __restrict_
                                                       uint64_t tiles, uint32_t* tile_cnt) {
   for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
     const uint64_t c0 = t * kTileBytes + uint64_t(threadIdx.x) * kChunkBytes;
+    uint32_t c = 0;
+    if (c0 < len) {
+      Chunk64 ch;
+      ch.load(text, c0);
+      c = count_starts(ch, uint32_t(min<uint64_t>(kChunkBytes, len - c0)), starts_at(text, c0));
+    }
     uint32_t total;
-    (void)block_exclusive_scan(c0 < len ? count_starts(text, len, c0) : 0u, &total);
+    (void)block_exclusive_scan(c, &total);
     if (threadIdx.x == 0) tile_cnt[t] = total;
   }
 }
 
-// Parses the line starting at q: x, y and its kind (0 two digit runs, 1 no
-// digit, 2 one digit run).
-__device__ __forceinline__ uint32_t parse_line(const uint8_t* __restrict__ text, uint64_t len, uint64_t q,
-                                               uint32_t* x_out, uint32_t* y_out) {
-  uint64_t j = q;
-  while (j < len && text[j] != '\n' && !is_digit(text[j])) ++j;
-  if (j == len || text[j] == '\n') return 1;
-  uint32_t x = 0, y = 0;
-  while (j < len && is_digit(text[j])) x = x * 10u + uint32_t(int8_t(text[j++]) - '0');
-  while (j < len && text[j] != '\n' && !is_digit(text[j])) ++j;
-  if (j == len || text[j] == '\n') return 2;
-  while (j < len && is_digit(text[j])) y = y * 10u + uint32_t(int8_t(text[j++]) - '0');
-  *x_out = x;
-  *y_out = y;
-  return 0;
-}
+// Per-line state machine of the contract: 0 before x, 1 in x, 2 between x
+// and y, 3 in y, 4 after y. A line's kind at its end: 0 two digit runs
+// (states 3, 4), 1 no digit (0), 2 one digit run (1, 2).
+struct LineParse {
+  uint32_t state = 0, x = 0, y = 0;
+  __device__ __forceinline__ void step(uint8_t c) {
+    const bool digit = int8_t(c) >= '0';
+    const uint32_t d = uint32_t(int8_t(c) - '0');
+    switch (state) {
+      case 0: if (digit) { state = 1; x = d; } break;
+      case 1: if (digit) x = x * 10u + d; else state = 2; break;
+      case 2: if (digit) { state = 3; y = d; } break;
+      case 3: if (digit) y = y * 10u + d; else state = 4; break;
+      default: break;
+    }
+  }
+  __device__ __forceinline__ uint32_t kind() const { return state >= 3 ? 0u : state == 0 ? 1u : 2u; }
+};
+
+struct LineOut {
+  uint32_t* xs;
+  uint32_t* ys;
+  uint8_t* kind;
+  uint32_t mx = 0, my = 0, blank = 0, single = 0;
+  __device__ __forceinline__ void emit(uint64_t at, const LineParse& lp) {
+    const uint32_t k = lp.kind();
+    xs[at] = k == 0 ? lp.x : 0u;
+    ys[at] = k == 0 ? lp.y : 0u;
+    kind[at] = uint8_t(k);
+    if (k == 0) {
+      mx = max(mx, lp.x);
+      my = max(my, lp.y);
+    }
+    blank += k == 1;
+    single += k == 2;
+  }
+};
 
 // Every line starting in a thread's chunk, in file order: xs / ys / kind at
-// line index tile_off[t] + (starts before it in the tile). stat[0] and
-// stat[1]: largest x and y of two-run lines; stat[2]: lines of no digit;
-// stat[3]: lines of one digit run (stat starts at 0).
+// line index tile_off[t] + (starts before it in the tile). The chunk is
+// parsed from registers; the line still open at its end continues from
+// memory. stat[0] and stat[1]: largest x and y of two-run lines; stat[2]:
+// lines of no digit; stat[3]: lines of one digit run (stat starts at 0).
 __global__ __launch_bounds__(kBlock) void k_tsv_parse(const uint8_t* __restrict__ text, uint64_t len, uint64_t tiles,
                                                       const uint64_t* __restrict__ tile_off, uint32_t* xs,
                                                       uint32_t* ys, uint8_t* kind, uint32_t* stat) {
-  uint32_t mx = 0, my = 0, blank = 0, single = 0;
+  LineOut out{xs, ys, kind};
   for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
     const uint64_t c0 = t * kTileBytes + uint64_t(threadIdx.x) * kChunkBytes;
+    Chunk64 ch;
+    uint32_t nb = 0, cnt = 0;
+    bool open = false;  // a line that started in this chunk is being parsed
+    if (c0 < len) {
+      ch.load(text, c0);
+      nb = uint32_t(min<uint64_t>(kChunkBytes, len - c0));
+      open = starts_at(text, c0);
+      cnt = count_starts(ch, nb, open);
+    }
     uint32_t total;
-    const uint32_t before = block_exclusive_scan(c0 < len ? count_starts(text, len, c0) : 0u, &total);
+    uint64_t at = tile_off[t] + block_exclusive_scan(cnt, &total);
     if (c0 >= len) continue;
-    uint64_t at = tile_off[t] + before;
-    const uint64_t c1 = min(c0 + kChunkBytes, len);
-    uint8_t prev = c0 ? text[c0 - 1] : uint8_t('\n');
-    for (uint64_t q = c0; q < c1; ++q) {
-      const bool start = prev == '\n';
-      prev = text[q];
-      if (!start) continue;
-      uint32_t x = 0, y = 0;
-      const uint32_t k = parse_line(text, len, q, &x, &y);
-      xs[at] = x;
-      ys[at] = y;
-      kind[at] = uint8_t(k);
-      ++at;
-      if (k == 0) {
-        mx = max(mx, x);
-        my = max(my, y);
+    LineParse lp;
+#pragma unroll
+    for (int i = 0; i < int(kChunkBytes); ++i) {
+      if (uint32_t(i) < nb) {
+        const uint8_t c = ch.byte(i);
+        if (c == '\n') {
+          if (open) out.emit(at++, lp);
+          lp = LineParse{};
+          open = uint32_t(i) + 1 < nb;  // the next line starts here unless the chunk or file ends
+        } else if (open) {
+          lp.step(c);
+        }
       }
-      blank += k == 1;
-      single += k == 2;
+    }
+    if (open) {  // the last line runs past the chunk: finish it from memory
+      uint64_t j = c0 + nb;
+      while (j < len && text[j] != '\n') lp.step(text[j++]);
+      out.emit(at++, lp);
     }
   }
   // one atomic per wave
   for (int o = kWave / 2; o > 0; o >>= 1) {
-    mx = max(mx, uint32_t(__shfl_xor(mx, o)));
-    my = max(my, uint32_t(__shfl_xor(my, o)));
-    blank += uint32_t(__shfl_xor(blank, o));
-    single += uint32_t(__shfl_xor(single, o));
+    out.mx = max(out.mx, uint32_t(__shfl_xor(out.mx, o)));
+    out.my = max(out.my, uint32_t(__shfl_xor(out.my, o)));
+    out.blank += uint32_t(__shfl_xor(out.blank, o));
+    out.single += uint32_t(__shfl_xor(out.single, o));
   }
   if (lane64() == 0) {
-    atomicMax(&stat[0], mx);
-    atomicMax(&stat[1], my);
-    if (blank) atomicAdd(&stat[2], blank);
-    if (single) atomicAdd(&stat[3], single);
+    atomicMax(&stat[0], out.mx);
+    atomicMax(&stat[1], out.my);
+    if (out.blank) atomicAdd(&stat[2], out.blank);
+    if (out.single) atomicAdd(&stat[3], out.single);
   }
 }
 
@@ -160,20 +218,38 @@ bool Solver::ingest_tsv(const char* path, uint64_t* lines, uint32_t* n) {
   // file -> pinned staging (two buffers, alternating) -> HBM
   for (int b = 0; b < 2; ++b)
     if (!stage_[b]) KT_HIP(hipHostMalloc(reinterpret_cast<void**>(&stage_[b]), kStageBytes, hipHostMallocDefault));
+  // a chunk of the file into a staging buffer, read by a few threads (one
+  // core does not keep up with the copy engine)
+  auto read_into = [&](uint8_t* dst, uint64_t at, size_t want) {
+    constexpr size_t kReaders = 4;
+    const size_t part = (want + kReaders - 1) / kReaders;
+    std::atomic<bool> ok{true};
+    std::thread th[kReaders];
+    for (size_t r = 0; r < kReaders; ++r) {
+      const size_t a = std::min(want, r * part), e = std::min(want, a + part);
+      th[r] = std::thread([&, a, e] {
+        for (size_t got = a; got < e;) {
+          const ssize_t n = ::pread(file.fd, dst + got, e - got, off_t(at + got));
+          if (n <= 0) {
+            ok = false;
+            return;
+          }
+          got += size_t(n);
+        }
+      });
+    }
+    for (auto& x : th) x.join();
+    return ok.load();
+  };
   uint64_t off = 0;
   int b = 0;
   bool staged[2] = {false, false};
   while (off < len) {
     if (staged[b]) KT_HIP(hipEventSynchronize(stage_ev_[b]));
     const size_t want = size_t(std::min<uint64_t>(kStageBytes, len - off));
-    size_t got = 0;
-    while (got < want) {
-      const ssize_t r = ::pread(file.fd, stage_[b] + got, want - got, off_t(off + got));
-      if (r <= 0) {
-        KT_HIP(hipStreamSynchronize(stream_));
-        fail(std::string("cannot read ") + path);
-      }
-      got += size_t(r);
+    if (!read_into(stage_[b], off, want)) {
+      KT_HIP(hipStreamSynchronize(stream_));
+      fail(std::string("cannot read ") + path);
     }
     KT_HIP(hipMemcpyAsync(text + off, stage_[b], want, hipMemcpyHostToDevice, stream_));
     KT_HIP(hipEventRecord(stage_ev_[b], stream_));
