@@ -682,12 +682,36 @@ __global__ __launch_bounds__(kBlock) void k_orient_fill(
 // are skipped (the host routes them to a segmented radix sort).
 constexpr uint32_t kSortCap = 8192;
 
+// Rows of 2..64 entries: one wave per row, a bitonic network over the lanes
+// (registers and shuffles only; a workgroup per short row would spend most
+// of its time in barriers).
+__global__ __launch_bounds__(kBlock) void k_sort_rows_wave(const uint32_t* __restrict__ off, uint32_t nrows,
+                                                           uint32_t* vals) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; row < nrows; row += waves) {
+    const uint32_t b = off[row], len = off[row + 1] - b;
+    if (len < 2 || len > uint32_t(kWave)) continue;  // wave-uniform
+    uint32_t v = uint32_t(lane) < len ? vals[b + lane] : kNone;
+#pragma unroll
+    for (int size = 2; size <= kWave; size <<= 1) {
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const uint32_t p = __shfl_xor(v, stride);
+        const bool up = (lane & size) == 0, low = (lane & stride) == 0;
+        v = (low == up) ? min(v, p) : max(v, p);
+      }
+    }
+    if (uint32_t(lane) < len) vals[b + lane] = v;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_sort_rows(const uint32_t* __restrict__ off,
                                                       uint32_t nrows, uint32_t* vals) {
   __shared__ uint32_t buf[kSortCap];
   for (uint32_t row = blockIdx.x; row < nrows; row += gridDim.x) {
     const uint32_t b = off[row], len = off[row + 1] - b;
-    if (len < 2 || len > kSortCap) continue;  // block-uniform
+    if (len <= uint32_t(kWave) || len > kSortCap) continue;  // block-uniform; short rows: k_sort_rows_wave
     uint32_t P = 2;
     while (P < len) P <<= 1;
     for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = (i < len) ? vals[b + i] : kNone;
@@ -1303,8 +1327,9 @@ void orient_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* to, c
                                                                     src, din);
 }
 void sort_rows(const uint32_t* off, uint32_t nrows, uint32_t* vals, uint32_t max_row, hipStream_t s) {
-  (void)max_row;
-  if (nrows) k_sort_rows<<<nrows < 65536 ? nrows : 65536, kBlock, 0, s>>>(off, nrows, vals);
+  if (!nrows) return;
+  k_sort_rows_wave<<<std::min(grid_for(uint64_t(nrows) * kWave), 16384), kBlock, 0, s>>>(off, nrows, vals);
+  if (max_row > uint32_t(kWave)) k_sort_rows<<<nrows < 65536 ? nrows : 65536, kBlock, 0, s>>>(off, nrows, vals);
 }
 void build_undirected(const uint32_t* off_d, const uint32_t* adj_d, const uint32_t* din_off,
                       const uint32_t* in_src_sorted, const uint32_t* in_eid_sorted, uint32_t nc,
