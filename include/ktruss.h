@@ -54,6 +54,8 @@ typedef struct kt_result {
   float ms_count;         /* (c) pass-2 support count */
   float ms_peel;          /* (d) pass-2 edge peeling */
   float ms_count_kernel;  /* device time of the pass-2 support kernel alone */
+  uint32_t compactions;   /* edge-set compactions, both passes (src/kc.cu:1059-1138) */
+  uint32_t recounts;      /* dropped frontiers recounted exactly (src/kc.cu:1206-1233) */
 } kt_result;
 
 /* Byte-model terms of the pass-2 core of the last solve: C is the
@@ -112,8 +114,10 @@ int kt_truss_edges(kt_context* ctx, uint32_t* pairs, uint64_t cap, uint64_t* cou
  * full truss decomposition on the device, then (u, v, t) triples, u < v in
  * original ids, for every edge of trussness t >= 3 (every other edge has
  * trussness 2), unordered. *count receives how many; at most cap triples
- * are written. Afterwards kt_truss_edges has no survivors to report until
- * the next solve, so call it first. */
+ * are written. The decomposition runs once per input graph: a second call
+ * (e.g. with a buffer sized by the first call's count) only copies. The last
+ * solve's kmax-truss stays available to kt_truss_edges; kt_last_stats fails
+ * afterwards (the decomposition reuses the core buffers it describes). */
 int kt_trussness(kt_context* ctx, uint32_t* triples, uint64_t cap, uint64_t* count);
 
 /* Host TSV parser (the reference's input contract, src/kc_gpu.cpp:243-312).

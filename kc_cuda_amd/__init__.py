@@ -52,6 +52,7 @@ class KtResult(ctypes.Structure):
         ("ms_total", ctypes.c_float), ("ms_build", ctypes.c_float), ("ms_kcore", ctypes.c_float),
         ("ms_pass1", ctypes.c_float), ("ms_extract", ctypes.c_float), ("ms_count", ctypes.c_float),
         ("ms_peel", ctypes.c_float), ("ms_count_kernel", ctypes.c_float),
+        ("compactions", ctypes.c_uint32), ("recounts", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -303,8 +304,9 @@ class Context:
 
     def trussness(self) -> np.ndarray:
         """(u, v, t) rows, u < v, sorted, for every edge of the last solved
-        graph with trussness t >= 3 (the others have 2). Replaces the last
-        solve's survivors: call truss_edges() first."""
+        graph with trussness t >= 3 (the others have 2). The decomposition
+        runs once (the second C call only copies); truss_edges() still
+        returns the last solve's kmax-truss afterwards."""
         cnt = _U64()
         _check(self._lib.kt_trussness(self._h, None, 0, ctypes.byref(cnt)))
         out = np.zeros((max(cnt.value, 1), 3), dtype=np.uint32)

@@ -110,6 +110,7 @@ void Solver::core_ranks_to_host(const CoreView& cv, uint32_t* inv_orig, uint32_t
 // reference's build_nbr_direct, src/kc.cu:1478-1580): row to[u] holds
 // {to[v] : v kept, to[v] > to[u]}, ascending, at the caller's offsets.
 void Solver::orient_with_ranks(const uint32_t* h_to, uint32_t new_n, const uint32_t* h_off_d, uint32_t* h_adj_d) {
+  core_valid_ = false;  // the core buffers are rewritten
   LevelGraph& g = *levels_[0];
   std::vector<uint32_t> l(g.n), to(g.n), inv(size_t(new_n) + 1, 0);
   KT_HIP(hipMemcpyAsync(l.data(), g.lvl, sizeof(uint32_t) * g.n, hipMemcpyDeviceToHost, stream_));
@@ -143,14 +144,14 @@ void Solver::orient_with_ranks(const uint32_t* h_to, uint32_t new_n, const uint3
   }
   orient_fill(g.off, g.adj, to_.as<uint32_t>(), inv_.as<uint32_t>(), new_n, off_d_.as<uint32_t>(),
               adj_d_.as<uint32_t>(), src_.as<uint32_t>(), nullptr, stream_);
-  if (mx > 8192) fail("build_nbr_direct: out-degree above 8192");
-  sort_rows(off_d_.as<uint32_t>(), new_n, adj_d_.as<uint32_t>(), mx, stream_);
+  sort_oriented_rows(new_n, mc, mx);
   if (mc) KT_HIP(hipMemcpyAsync(h_adj_d, adj_d_.p, sizeof(uint32_t) * mc, hipMemcpyDeviceToHost, stream_));
   KT_HIP(hipStreamSynchronize(stream_));
 }
 
 // Core view over a host oriented CSR in rank ids (rows ascending, u < v).
 void Solver::view_from_oriented_host(uint32_t nc, const uint32_t* h_off_d, const uint32_t* h_adj_d, CoreView* cv) {
+  core_valid_ = false;  // the core buffers are rewritten
   const uint32_t mc = h_off_d[nc];
   if (mc >= 0x7FFFFFFFu) fail("oriented graph exceeds 2^31 edges");
   uint32_t mx = 0;

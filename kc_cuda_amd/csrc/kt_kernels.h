@@ -52,6 +52,14 @@ void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t
 // lvl[v] = (lvl[v] > k) ? kNone : (lvl[v] == kNone ? kNone : k) -- used to
 // re-open every vertex whose removal threshold exceeds k.
 void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s);
+// Degree histogram (kDegBins bins: exact below kDegExact, then one per power
+// of two) and the degree filter of the max-core search (lvl = k for every
+// vertex of degree < k, kNone otherwise; *alive += vertices kept).
+constexpr uint32_t kDegExactLog2 = 12;
+constexpr uint32_t kDegExact = 1u << kDegExactLog2;
+constexpr uint32_t kDegBins = kDegExact + 32 - kDegExactLog2;
+void degree_hist(const uint32_t* deg, uint32_t n, uint32_t* hist, hipStream_t s);
+void core_filter(const uint32_t* deg, uint32_t n, uint32_t k, uint32_t* lvl, uint32_t* alive, hipStream_t s);
 
 // (a1-a3) core subgraph extraction.
 // key = degree << 32 | ~original id (to_orig may be null: identity), val = local id

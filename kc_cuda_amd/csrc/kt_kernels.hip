@@ -454,6 +454,40 @@ __global__ __launch_bounds__(kBlock) void k_core_alive_deg(const uint64_t* __res
   }
 }
 
+// Degree histogram for the start of the max-core search: exact bins for
+// degrees below kDegExact, one bin per power of two above (bin kDegExact + j
+// holds degrees in [2^(j + log2 kDegExact), 2^(j + 1 + log2 kDegExact))).
+// Counted per workgroup in LDS, merged with one atomic per non-empty bin.
+__global__ __launch_bounds__(kBlock) void k_degree_hist(const uint32_t* __restrict__ deg, uint32_t n,
+                                                        uint32_t* hist) {
+  __shared__ uint32_t s_h[kDegBins];
+  for (uint32_t i = threadIdx.x; i < kDegBins; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t d = deg[v];
+    const uint32_t b = d < kDegExact ? d : kDegExact + (31u - __clz(d)) - kDegExactLog2;
+    atomicAdd(&s_h[b], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kDegBins; i += blockDim.x)
+    if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// Vertices of degree < k cannot be in the k-core: stamp them removed at k
+// (no decrements: their neighbours' degrees are recounted afterwards) and
+// count the rest.
+__global__ __launch_bounds__(kBlock) void k_core_filter(const uint32_t* __restrict__ deg, uint32_t n, uint32_t k,
+                                                        uint32_t* lvl, uint32_t* alive) {
+  uint32_t c = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const bool keep = deg[v] >= k;
+    lvl[v] = keep ? kNone : k;
+    c += keep;
+  }
+  c = wave_sum(c);
+  if (lane64() == 0 && c) atomicAdd(alive, c);
+}
+
 __global__ void k_core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     const uint32_t l = lvl[v];
@@ -1274,6 +1308,13 @@ void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t
     k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, nullptr, hl.slices, nullptr);
     k_core_alive_deg_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, lvl, hl.items, hl.count, hl.slices, deg);
   }
+}
+void degree_hist(const uint32_t* deg, uint32_t n, uint32_t* hist, hipStream_t s) {
+  fill_u32(hist, kDegBins, 0, s);
+  if (n) k_degree_hist<<<std::min(grid_for(n), 512), kBlock, 0, s>>>(deg, n, hist);
+}
+void core_filter(const uint32_t* deg, uint32_t n, uint32_t k, uint32_t* lvl, uint32_t* alive, hipStream_t s) {
+  if (n) k_core_filter<<<std::min(grid_for(n), 4096), kBlock, 0, s>>>(deg, n, k, lvl, alive);
 }
 void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s) {
   if (n) k_core_reopen_above<<<grid_for(n), kBlock, 0, s>>>(lvl, n, k);

@@ -67,7 +67,8 @@ struct TrussOutcome {
   int32_t floor_thr = -1;  // survivors: del_thr == kNone or del_thr > floor_thr
   uint32_t levels = 0;
   uint64_t rounds = 0;
-  uint32_t compactions = 0;
+  uint32_t compactions = 0;  // edge sets rewritten (compaction, re-densify)
+  uint32_t recounts = 0;     // dropped frontiers followed by an exact recount
 };
 
 struct EdgeSet {
@@ -155,10 +156,12 @@ class Solver {
   // fresh: start from full degrees with every vertex alive; otherwise from
   // the state already on level 0. allow_cut: continue on induced subgraphs.
   uint32_t max_core(bool allow_cut = true, bool fresh = true);
+  bool filtered_start(LevelGraph** gp, uint32_t* k_out, uint64_t* alive_out);
   LevelGraph& level0();
   LevelGraph& cut_level(LevelGraph& parent, uint32_t k_enter, uint32_t alive);
   void core_at(uint32_t k);  // leaves the k-core alive on the deepest usable level
   void extract_core(LevelGraph& g, CoreView* cv);
+  void sort_oriented_rows(uint32_t nc, uint32_t mc, uint32_t max_row);
   void build_undirected_view(CoreView* cv);
   void count_support(CoreView* cv, float* ms);
   // Keep only the live edges (state == kNone) of the current view; with
@@ -229,7 +232,7 @@ class Solver {
   bool force_edge_count_ = false;
   bool force_atomic_build_ = false;
   int core_mode_ = 0;
-  DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_;
+  DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_, dhist_;
   DevBuf bk_state_, bk_sup_, bk_bits_, bk_del_;  // threshold-probe backups
   uint64_t eidm_budget_ = uint64_t(4) << 30;  // bytes for the edge-id matrix
   double dense_tri_cost_ = 4.0, sparse_drop_ratio_ = 3.0, default_drop_ratio_ = 3.0;
@@ -267,7 +270,14 @@ class Solver {
   CoreView last_core_{};
   int32_t last_floor_ = -1;
   uint64_t last_truss_ = 0;
-  bool have_truss_ = false;
+  bool have_truss_ = false;    // a solve completed on the current input
+  bool core_valid_ = false;    // last_core_ still describes that solve's core buffers
+  bool truss_ready_ = false;   // its kmax-truss pairs are in truss_out_
+  bool triples_ready_ = false; // the current input's trussness triples are in out_pairs_
+  uint64_t truss_count_ = 0, triples_count_ = 0;
+  DevBuf truss_out_;
+  void new_input();            // a new graph: results of the previous one are gone
+  void materialize_truss();
 };
 
 }  // namespace kt

@@ -57,9 +57,9 @@ void Solver::each_buffer(F f) const {
   const DevBuf* all[] = {&off_, &adj_, &deg_, &lvl_, &q0_, &q1_, &deg_bak_, &cursor_, &list_, &new_id_,
                          &keys_, &keys2_, &vals_, &vals2_, &to_, &inv_, &inv_orig_, &dout_, &off_d_,
                          &adj_d_, &src_, &din_, &din_off_, &tkeys_, &tvals_, &tiota_, &in_src_, &off_u_,
-                         &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &del_thr_, &orig0_,
+                         &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &truss_out_, &del_thr_, &orig0_,
                          &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &fbits_, &pref0_, &eidm_, &heavy_,
-                         &vflag_, &vmap_, &roff_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_, &text_,
+                         &vflag_, &vmap_, &roff_, &dhist_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_, &text_,
                          &tile_cnt_, &tile_off_, &lx_, &ly_, &lkind_, &lx2_, &ly2_, &lkeep_, &lkeep_off_,
                          &lstat_, &probe_stats_};
   for (const DevBuf* b : all) f(*b);
@@ -182,7 +182,14 @@ bool Solver::build_partitioned(int mode, const uint32_t* a, const uint32_t* b, u
   return true;
 }
 
+void Solver::new_input() {
+  have_truss_ = false;
+  truss_ready_ = false;
+  triples_ready_ = false;
+}
+
 void Solver::build_from_pairs(const uint32_t* d_pairs, uint64_t m, uint32_t n) {
+  new_input();
   if (reinterpret_cast<uintptr_t>(d_pairs) % 8 != 0) fail("pairs must be 8-byte aligned");
   if (2 * m >= (uint64_t(1) << 40)) fail("edge list too large");
   n_ = n;
@@ -203,6 +210,7 @@ void Solver::build_from_pairs(const uint32_t* d_pairs, uint64_t m, uint32_t n) {
 }
 
 void Solver::build_from_lines(const uint32_t* d_xs, const uint32_t* d_ys, uint64_t L, uint32_t n) {
+  new_input();
   n_ = n;
   m_ = L / 2;
   deg_.ensure(sizeof(uint32_t) * (size_t(n) + 1));
@@ -221,6 +229,7 @@ void Solver::build_from_lines(const uint32_t* d_xs, const uint32_t* d_ys, uint64
 }
 
 void Solver::load_csr_host(uint32_t n, const uint64_t* off, const uint32_t* adj) {
+  new_input();
   n_ = n;
   m_ = off[n] / 2;
   deg_.ensure(sizeof(uint32_t) * (size_t(n) + 1));

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <vector>
 
 #include "kt_batch_union.h"
 #include "kt_kernels.h"
@@ -123,6 +124,67 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
   return s;
 }
 
+// Start of the max-core search from a high threshold. The k-core lies
+// inside the subgraph induced by the vertices of degree >= k (and is that
+// subgraph's k-core), so a threshold below the max core is reached by one
+// filter of the vertices, one recount of the kept rows and a cut, and the
+// peel then runs on the small induced subgraph instead of cascading through
+// the whole graph from threshold 1. The threshold is an eighth of a lower
+// bound of the degree h-index H (at least H vertices have degree >= H; the
+// max core is at most H); when its core is empty the threshold drops by 4x,
+// and when the filter would keep more than a quarter of the vertices the
+// plain search from 1 runs instead. On success *gp is the cut level with the
+// kf-core alive, *k_out = kf and *alive_out its vertex count.
+bool Solver::filtered_start(LevelGraph** gp, uint32_t* k_out, uint64_t* alive_out) {
+  LevelGraph& g0 = **gp;
+  const uint32_t n = g0.n;
+  if (n < 4 * kMinCut) return false;
+  dhist_.ensure(sizeof(uint32_t) * kDegBins);
+  degree_hist(g0.deg, n, dhist_.as<uint32_t>(), stream_);
+  std::vector<uint32_t> h(kDegBins);
+  KT_HIP(hipMemcpyAsync(h.data(), dhist_.p, sizeof(uint32_t) * kDegBins, hipMemcpyDeviceToHost, stream_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  // at_least[b]: vertices whose degree is at least the lower edge of bin b
+  std::vector<uint64_t> at_least(kDegBins + 1, 0);
+  uint64_t hidx = 0;
+  for (int b = int(kDegBins) - 1; b >= 0; --b) {
+    at_least[b] = at_least[b + 1] + h[b];
+    const uint64_t lo = uint32_t(b) < kDegExact ? uint64_t(b) : uint64_t(1) << (b - kDegExact + kDegExactLog2);
+    hidx = std::max(hidx, std::min(at_least[b], lo));
+  }
+  auto kept_at_least = [&](uint32_t k) {  // vertices of degree >= k (exact below kDegExact, else a lower bound)
+    if (k < kDegExact) return at_least[k];
+    uint32_t b = kDegExact + (31u - __builtin_clz(k)) - kDegExactLog2;
+    if ((uint64_t(1) << (b - kDegExact + kDegExactLog2)) < k) ++b;
+    return at_least[std::min<uint32_t>(b, kDegBins)];
+  };
+  for (uint32_t kf = uint32_t(hidx / 8); kf >= 3; kf /= 4) {
+    if (kept_at_least(kf) * 4 > n) return false;
+    zero_u32(&ctl_->nalive);
+    core_filter(g0.deg, n, kf, g0.lvl, &ctl_->nalive, stream_);
+    const uint32_t kept = read_u32(&ctl_->nalive);
+    if (kept > 0) {
+      core_alive_degrees(g0.off, g0.adj, g0.lvl, n, g0.deg, heavy(), stream_);
+      LevelGraph& s = cut_level(g0, kf, kept);
+      const uint64_t removed = peel_vertices(s, kf);
+      if (trace_)
+        std::fprintf(stderr, "[kt] max-core filter k=%u (h-index >= %llu): %u kept, %llu in the core\n", kf,
+                     (unsigned long long)hidx, kept, (unsigned long long)(kept - std::min<uint64_t>(removed, kept)));
+      if (removed < kept) {
+        *gp = &s;
+        *k_out = kf;
+        *alive_out = kept - removed;
+        return true;
+      }
+    }
+    // the kf-core is empty: back to the whole graph
+    depth_ = 1;
+    degrees_from_offsets(g0.off, n, g0.deg, stream_);
+    fill_u32(g0.lvl, n, kNone, stream_);
+  }
+  return false;
+}
+
 // Galloping search for the largest k with a non-empty k-core. Leaves the
 // max-core alive on the deepest level and returns its k (0: no edges).
 uint32_t Solver::max_core(bool allow_cut, bool fresh) {
@@ -140,13 +202,16 @@ uint32_t Solver::max_core(bool allow_cut, bool fresh) {
     alive_list(g->lvl, g->n, q1_.as<uint32_t>(), q0_.as<uint32_t>(), &ctl_->nalive, stream_);
     before = read_u32(&ctl_->nalive);
   }
-  const uint64_t gone = peel_vertices(*g, 1);
-  if (gone >= before) return 0;
-  uint64_t alive = before - gone;
   uint32_t k = 1, step = 16;
+  uint64_t alive = 0;
+  if (!(allow_cut && fresh && filtered_start(&g, &k, &alive))) {
+    const uint64_t gone = peel_vertices(*g, 1);
+    if (gone >= before) return 0;
+    alive = before - gone;
+    g->ok.assign(1, 1);
+    if (allow_cut && alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, 1, uint32_t(alive));
+  }
   bool grow = true;
-  g->ok.assign(1, 1);
-  if (allow_cut && alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, 1, uint32_t(alive));
   while (step > 0) {
     const uint32_t kk = k + step;
     KT_HIP(hipMemcpyAsync(deg_bak_.p, g->deg, sizeof(uint32_t) * g->n, hipMemcpyDeviceToDevice, stream_));
@@ -187,7 +252,28 @@ void Solver::core_at(uint32_t k) {
 
 // --------------------------------------------------- (a1-a3) extraction ---
 
+// Rows of the oriented CSR in off_d_ / adj_d_ ascending: rows of up to 8192
+// entries in LDS (one wave or one workgroup per row), longer rows by a
+// segmented radix sort over all rows.
+void Solver::sort_oriented_rows(uint32_t nc, uint32_t mc, uint32_t max_row) {
+  if (max_row <= 8192) {
+    sort_rows(off_d_.as<uint32_t>(), nc, adj_d_.as<uint32_t>(), max_row, stream_);
+    return;
+  }
+  tkeys_.ensure(sizeof(uint32_t) * std::max<uint32_t>(mc, 1));
+  size_t tb = 0;
+  KT_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, adj_d_.as<uint32_t>(), tkeys_.as<uint32_t>(), mc,
+                                                    nc, off_d_.as<uint32_t>(), off_d_.as<uint32_t>() + 1, 0, 32,
+                                                    stream_));
+  temp_.ensure(tb);
+  KT_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(temp_.p, tb, adj_d_.as<uint32_t>(), tkeys_.as<uint32_t>(), mc,
+                                                    nc, off_d_.as<uint32_t>(), off_d_.as<uint32_t>() + 1, 0, 32,
+                                                    stream_));
+  KT_HIP(hipMemcpyAsync(adj_d_.p, tkeys_.p, sizeof(uint32_t) * mc, hipMemcpyDeviceToDevice, stream_));
+}
+
 void Solver::extract_core(LevelGraph& g, CoreView* cv) {
+  core_valid_ = false;  // the core buffers are rewritten
   const uint32_t n = g.n;
   const size_t nn = size_t(n_) + 1;
   keys_.ensure(sizeof(uint64_t) * nn);
@@ -220,21 +306,7 @@ void Solver::extract_core(LevelGraph& g, CoreView* cv) {
   src_.ensure(sizeof(uint32_t) * mm);
   orient_fill(g.off, g.adj, to_.as<uint32_t>(), inv_.as<uint32_t>(), nc, off_d_.as<uint32_t>(),
               adj_d_.as<uint32_t>(), src_.as<uint32_t>(), nullptr, stream_);
-  // rows ascending
-  if (cv->max_dout <= 8192) {
-    sort_rows(off_d_.as<uint32_t>(), nc, adj_d_.as<uint32_t>(), cv->max_dout, stream_);
-  } else {
-    tkeys_.ensure(sizeof(uint32_t) * mm);
-    size_t tb = 0;
-    KT_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, adj_d_.as<uint32_t>(), tkeys_.as<uint32_t>(),
-                                                      mc, nc, off_d_.as<uint32_t>(), off_d_.as<uint32_t>() + 1,
-                                                      0, 32, stream_));
-    temp_.ensure(tb);
-    KT_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(temp_.p, tb, adj_d_.as<uint32_t>(), tkeys_.as<uint32_t>(),
-                                                      mc, nc, off_d_.as<uint32_t>(), off_d_.as<uint32_t>() + 1,
-                                                      0, 32, stream_));
-    KT_HIP(hipMemcpyAsync(adj_d_.p, tkeys_.p, sizeof(uint32_t) * mc, hipMemcpyDeviceToDevice, stream_));
-  }
+  sort_oriented_rows(nc, mc, cv->max_dout);
   orig0_.ensure(sizeof(uint32_t) * mm);
   del_thr_.ensure(sizeof(uint32_t) * mm);
   sup_.ensure(sizeof(int32_t) * mm);
@@ -522,9 +594,11 @@ TrussOutcome Solver::run_levels(CoreView* cv, int32_t thr, uint64_t alive, uint3
     alive = h.alive;
     thr = h.thr;
     if (h.action == kRoundDrop) {  // drop the stamped frontier, recount the rest exactly
+      ++out.recounts;
       if (cv->dense) {
         if (allow_redensify && try_redensify(cv)) {  // smaller matrix on the vertices left, exact recount
           if (cv->mc != alive) fail("edge peel bookkeeping mismatch (redensify)");
+          ++out.compactions;
           r = 1;
         } else {
           frontier_clear(peel_args(*cv), ctl_, stream_);
@@ -532,17 +606,18 @@ TrussOutcome Solver::run_levels(CoreView* cv, int32_t thr, uint64_t alive, uint3
         }
       } else {
         compact_core(cv, false);
+        ++out.compactions;
         alive = cv->mc;
         r = 1;
       }
     } else if (h.action == kRoundCompact) {  // half the edges are dead: compact, keep supports
       compact_core(cv, true);
+      ++out.compactions;
       if (cv->mc != alive) fail("edge peel bookkeeping mismatch (compaction)");
       r = 1;
     } else {
       fail("edge peel: unexpected round action");
     }
-    ++out.compactions;
     start_rounds(cv, thr, r, alive, false, search);
     batch = 2;
     queued = 0;
@@ -576,6 +651,7 @@ TrussOutcome Solver::truss_gallop(CoreView* cv, int64_t empty_from) {
   out.rounds += lv.rounds;
   out.levels += lv.levels;
   out.compactions += lv.compactions;
+  out.recounts += lv.recounts;
   out.kmax = lv.kmax;
   out.count = lv.count;
   out.floor_thr = lv.floor_thr;
@@ -616,6 +692,7 @@ TrussOutcome Solver::truss_gallop(CoreView* cv, int64_t empty_from) {
       out.rounds += w.rounds;
       out.levels += w.levels;
       out.compactions += w.compactions;
+      out.recounts += w.recounts;
       if (w.kmax >= 2 && w.count > 0) {
         out.kmax = w.kmax;
         out.count = w.count;
@@ -634,6 +711,7 @@ TrussOutcome Solver::truss_gallop(CoreView* cv, int64_t empty_from) {
     out.rounds += p.rounds;
     out.levels += 1;
     out.compactions += p.compactions;
+    out.recounts += p.recounts;
     if (p.alive > 0 && p.kmax >= 2) {
       lo = int64_t(p.kmax) - 2;  // >= t
       alive = p.alive;
@@ -710,6 +788,7 @@ void Solver::solve_csr(kt_result* res) {
     KT_HIP(hipEventElapsedTime(&res->ms_build, ev_[0], ev_[1]));
     KT_HIP(hipEventElapsedTime(&res->ms_kcore, ev_[1], ev_[2]));
     have_truss_ = true;
+    core_valid_ = true;  // empty view
     return;
   }
 
@@ -743,6 +822,8 @@ void Solver::solve_csr(kt_result* res) {
   res->kmax = p2.kmax;
   res->truss_edges = p2.kmax ? p2.count : 0;
   res->peel_rounds = p1.rounds + p2.rounds;
+  res->compactions = p1.compactions + p2.compactions;
+  res->recounts = p1.recounts + p2.recounts;
   res->levels = p1.levels + p2.levels;
   float t_a = 0, t_b = 0;
   KT_HIP(hipEventElapsedTime(&res->ms_total, ev_[0], ev_[7]));
@@ -755,6 +836,7 @@ void Solver::solve_csr(kt_result* res) {
   KT_HIP(hipEventElapsedTime(&res->ms_count, ev_[5], ev_[6]));
   KT_HIP(hipEventElapsedTime(&res->ms_peel, ev_[6], ev_[7]));
   last_core_ = cv;
+  core_valid_ = true;
   last_floor_ = p2.floor_thr;
   last_truss_ = res->truss_edges;
   have_truss_ = true;
@@ -762,6 +844,7 @@ void Solver::solve_csr(kt_result* res) {
 
 void Solver::last_stats(kt_stats* out) {
   if (!have_truss_) fail("no solve has completed on this context");
+  if (!core_valid_) fail("the last solve's core view was replaced by a trussness computation");
   std::memset(out, 0, sizeof(*out));
   const CoreView& cv = last_core_;
   out->n_core = cv.nc0;
@@ -784,47 +867,66 @@ void Solver::last_stats(kt_stats* out) {
   out->w_peeled = h[1];
 }
 
+// The kmax-truss of the last solve as (u, v) pairs in input ids, written
+// once into truss_out_ (from the core view's deletion thresholds).
+void Solver::materialize_truss() {
+  if (truss_ready_) return;
+  if (!have_truss_) fail("no solve has completed on this context");
+  if (!core_valid_) fail("the last solve's core view is gone");
+  const CoreView& cv = last_core_;
+  truss_count_ = 0;
+  if (cv.mc0 && last_truss_) {
+    truss_out_.ensure(sizeof(uint32_t) * 2 * size_t(cv.mc0));
+    zero_u32(&ctl_->nalive);
+    truss_pairs(cv.del_thr, cv.mc0, last_floor_, cv.src0, cv.adj_d0, cv.inv, truss_out_.as<uint32_t>(),
+                &ctl_->nalive, stream_);
+    truss_count_ = read_u32(&ctl_->nalive);
+  }
+  truss_ready_ = true;
+}
+
 uint64_t Solver::trussness(uint32_t* host_triples, uint64_t cap) {
   KT_HIP(hipSetDevice(device_));
   if (!n_) return 0;
-  // full decomposition of the last input's CSR (level 0 is never modified
-  // after its build); the last solve's truss record is replaced
-  have_truss_ = false;
-  depth_ = 1;
-  LevelGraph& g = level0();
-  degrees_from_offsets(g.off, g.n, g.deg, stream_);
-  fill_u32(g.lvl, g.n, kNone, stream_);
-  peel_vertices(g, 2);  // a vertex of degree < 2 is in no triangle
-  CoreView cv;
-  extract_core(g, &cv);
-  if (!cv.mc0) return 0;
-  count_support(&cv, nullptr);
-  const uint32_t kall = truss_search(&cv, 1, 0, 0).kmax;
-  out_pairs_.ensure(sizeof(uint32_t) * 3 * size_t(cv.mc0));
-  zero_u32(&ctl_->nalive);
-  truss_triples(cv, kall, out_pairs_.as<uint32_t>(), &ctl_->nalive, stream_);
-  const uint32_t cnt = read_u32(&ctl_->nalive);
-  const uint64_t take = std::min<uint64_t>(cnt, cap);
+  if (!triples_ready_) {
+    // full decomposition of the last input's CSR (level 0 is never modified
+    // after its build); the core view buffers are reused, so the last
+    // solve's kmax-truss is saved first
+    if (have_truss_ && core_valid_) materialize_truss();
+    core_valid_ = false;
+    depth_ = 1;
+    LevelGraph& g = level0();
+    degrees_from_offsets(g.off, g.n, g.deg, stream_);
+    fill_u32(g.lvl, g.n, kNone, stream_);
+    peel_vertices(g, 2);  // a vertex of degree < 2 is in no triangle
+    CoreView cv;
+    extract_core(g, &cv);
+    triples_count_ = 0;
+    if (cv.mc0) {
+      count_support(&cv, nullptr);
+      const uint32_t kall = truss_search(&cv, 1, 0, 0).kmax;
+      out_pairs_.ensure(sizeof(uint32_t) * 3 * size_t(cv.mc0));
+      zero_u32(&ctl_->nalive);
+      truss_triples(cv, kall, out_pairs_.as<uint32_t>(), &ctl_->nalive, stream_);
+      triples_count_ = read_u32(&ctl_->nalive);
+    }
+    triples_ready_ = true;
+  }
+  const uint64_t take = std::min<uint64_t>(triples_count_, cap);
   if (take && host_triples)
     KT_HIP(hipMemcpyAsync(host_triples, out_pairs_.p, sizeof(uint32_t) * 3 * take, hipMemcpyDeviceToHost, stream_));
   KT_HIP(hipStreamSynchronize(stream_));
-  return cnt;
+  return triples_count_;
 }
 
 uint64_t Solver::truss_edges(uint32_t* host_pairs, uint64_t cap) {
-  if (!have_truss_) fail("no solve has completed on this context");
-  const CoreView& cv = last_core_;
-  if (cv.mc0 == 0 || last_truss_ == 0) return 0;
-  out_pairs_.ensure(sizeof(uint32_t) * 2 * size_t(cv.mc0));
-  zero_u32(&ctl_->nalive);
-  truss_pairs(cv.del_thr, cv.mc0, last_floor_, cv.src0, cv.adj_d0, cv.inv, out_pairs_.as<uint32_t>(),
-              &ctl_->nalive, stream_);
-  const uint32_t cnt = read_u32(&ctl_->nalive);
-  const uint64_t take = std::min<uint64_t>(cnt, cap);
+  KT_HIP(hipSetDevice(device_));
+  materialize_truss();
+  const uint64_t take = std::min<uint64_t>(truss_count_, cap);
   if (take && host_pairs)
-    KT_HIP(hipMemcpyAsync(host_pairs, out_pairs_.p, sizeof(uint32_t) * 2 * take, hipMemcpyDeviceToHost, stream_));
+    KT_HIP(hipMemcpyAsync(host_pairs, truss_out_.p, sizeof(uint32_t) * 2 * take, hipMemcpyDeviceToHost, stream_));
   KT_HIP(hipStreamSynchronize(stream_));
-  return cnt;
+  return truss_count_;
 }
 
 }  // namespace kt

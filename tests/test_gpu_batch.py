@@ -32,13 +32,19 @@ def _solve_batch(ctx, pairs, offs, ns, **kw):
 
 
 def _check_against_plain(ctx, pairs, offs, ns, res):
+    """Every graph's batched answer equals the plain HIP path's and the
+    oracle's (kmax, |T| and, with kept-pair flags, the kmax-truss)."""
+    from oracle import oracle as O
     for g in range(len(ns)):
         q = pairs[offs[g]:offs[g + 1]]
         r = ctx.solve_pairs(q, int(ns[g]))
-        assert (int(res.kmax[g]), int(res.edges[g])) == (r.kmax, r.truss_edges), f"graph {g} n={ns[g]}"
+        ref = O.solve_pairs(q, int(ns[g]), want_edges=True)
+        assert (r.kmax, r.truss_edges) == (ref.kmax, ref.truss_edges), f"graph {g} n={ns[g]} plain"
+        assert (int(res.kmax[g]), int(res.edges[g])) == (ref.kmax, ref.truss_edges), f"graph {g} n={ns[g]}"
         if res.keep is not None:
             kept = q[res.keep[offs[g]:offs[g + 1]] == 1]
             np.testing.assert_array_equal(canon_pairs(kept), ctx.truss_edges(), err_msg=f"graph {g}")
+            np.testing.assert_array_equal(canon_pairs(kept), canon_pairs(ref.truss), err_msg=f"graph {g}")
 
 
 @pytest.mark.parametrize("seed", [11, 12])

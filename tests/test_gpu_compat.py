@@ -137,3 +137,49 @@ def test_reference_driver_linked_against_library(tmp_path):
     for fname, line in list(exp.items())[:6]:
         r = O.run_kc_cpu(GOLDEN / "tiny" / fname, threads=2, timeout=60, binary=O.KC_GPU_LINKED)
         assert f"kmax = {r.kmax}, Edges in kmax-truss = {r.truss_edges}." == line
+
+
+def test_long_rows_clique_k8250(ctx):
+    """K_8250: every oriented row but the last is longer than 8192 entries
+    before it shrinks, so build_nbr_direct takes the segmented-sort branch;
+    closed forms: rank r = 8249 - id (equal degrees, id descending), row r
+    holds ranks r+1..8249, every doubled support is 2 * 8248, and the whole
+    clique is its kmax-truss: (8250, 34,027,125). The native solve sees the
+    same graph as a dense core of 8250 vertices."""
+    from kc_cuda_amd import compat as C
+    n = 8250
+    m = n * (n - 1) // 2
+    u, v = np.triu_indices(n, 1)
+    pairs = np.stack([u, v], axis=1).astype(np.uint32)
+    del u, v
+    print(f"[gpu] K{n}: native solve, m={m}", flush=True)
+    r = ctx.solve_pairs(pairs, n)
+    assert (r.kmax, r.truss_edges) == (n, m)
+    assert r.max_core == n - 1 and r.n_core == n and r.m_core == m
+    # the reference's CSR: row y holds every other vertex, ascending
+    nbr_u = (np.arange(n + 1, dtype=np.uint64) * (n - 1)).astype(np.uint32)
+    full = np.tile(np.arange(n, dtype=np.uint32), n).reshape(n, n)
+    nbr = full[~np.eye(n, dtype=bool)].astype(np.uint32)
+    del full
+    deg = np.full(n, n - 1, dtype=np.uint32)
+    print(f"[gpu] K{n}: compat stages", flush=True)
+    mask = np.zeros(n, dtype=np.uint32)
+    assert C.kmax_core_gpu(n, nbr_u, nbr, deg, mask, NONE) == n - 2
+    assert not mask.any()
+    dwm = np.zeros(n, dtype=np.uint32)
+    to = np.zeros(n, dtype=np.uint32)
+    li, new_n, cdeg, ddir = C.count_deg_with_mask(nbr, nbr_u, n, mask, dwm, to)
+    assert new_n == n
+    np.testing.assert_array_equal(to, (n - 1 - np.arange(n)).astype(np.uint32))
+    np.testing.assert_array_equal(ddir, (n - 1 - np.arange(n)).astype(np.uint32))
+    nud = np.concatenate([[0], np.cumsum(ddir, dtype=np.uint64)]).astype(np.uint32)
+    nd = C.build_nbr_direct(nbr, nbr_u, n, mask, to, nud, new_n)
+    want = np.concatenate([np.arange(rk + 1, n, dtype=np.uint32) for rk in range(n)])
+    np.testing.assert_array_equal(nd, want)
+    del want
+    unbr, fa, ls, rs = C.build_graph_from_direct(nd, nud, new_n)
+    tc, tcd = C.tricount_direct(nud, nd, new_n, ls, rs)
+    assert tc.size == m and (tc == 2 * (n - 2)).all()
+    assert (tcd == 2 * (n - 2)).all()
+    print(f"[gpu] K{n}: compat two-pass driver", flush=True)
+    assert C.solve(n, nbr_u, nbr, deg) == (n, m)

@@ -21,9 +21,13 @@ def _mix(seed: int, count: int, median: int, sigma: float, cap: int):
 
 
 def _alone(ctx, qs):
+    """Stand-alone plain solves, each checked against the oracle."""
+    from oracle import oracle as O
     out = []
-    for p, n in qs:
+    for i, (p, n) in enumerate(qs):
         r = ctx.solve_pairs(p, n)
+        ref = O.solve_pairs(p, n)
+        assert (r.kmax, r.truss_edges) == (ref.kmax, ref.truss_edges), f"query {i} alone vs oracle"
         out.append((r.kmax, r.truss_edges))
     return out
 

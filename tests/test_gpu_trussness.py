@@ -53,6 +53,10 @@ def test_trussness_top_level_on_rmat16(ctx):
     t = ctx.trussness()
     top = t[t[:, 2] == r.kmax][:, :2]
     np.testing.assert_array_equal(top, surv)
+    # the decomposition does not take the solve's kmax-truss away, and a
+    # second export copies the same triples
+    np.testing.assert_array_equal(ctx.truss_edges(), surv)
+    np.testing.assert_array_equal(ctx.trussness(), t)
     assert int(t[:, 2].max()) == r.kmax
     canon = canon_pairs(pairs)
     key = canon[:, 0].astype(np.uint64) << np.uint64(32) | canon[:, 1]
