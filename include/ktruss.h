@@ -254,6 +254,28 @@ int kt_last_stats(kt_context* ctx, kt_stats* out);
  * (off has n+1 entries, adj 2m). */
 int kt_last_csr(kt_context* ctx, uint64_t* off, uint32_t* adj);
 
+/* Allocation-free solve. No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The same solve and
+ * results as kt_solve_pairs_device, but every device buffer it needs is
+ * carved from a fixed scratch array of KT_NOALLOC_SCRATCH_BYTES per GPU (a
+ * file-scope __device__ array of the library, held on each GPU it runs on
+ * until the process ends); no context is needed and no hipMalloc, hipFree
+ * or allocating library routine is called. The first call on a device sets
+ * up that device's solver (its stream and timing events). A graph is
+ * admitted when 160*m + 64*n + min(72*m, 32 MiB) + min(512*m, 32 MiB) +
+ * 16 MiB <= KT_NOALLOC_SCRATCH_BYTES - 4 KiB (about 1.9M edges on few
+ * vertices); larger ones are rejected with an error before any work. When
+ * d_truss is non-null, the kmax-truss pairs (input ids, u < v, unordered;
+ * at most cap of them) are written there (device memory, 2*cap uint32);
+ * *truss_count (nullable) receives the truss size. Calls on one device are
+ * serialised in the process; calls from several processes on one device
+ * share its array and race. Returns when the results are written. */
+#define KT_NOALLOC_SCRATCH_BYTES (512ull << 20)
+int kt_solve_pairs_device_noalloc(int device, const uint32_t* d_pairs, uint64_t m, uint32_t n,
+                                  uint32_t* d_truss, uint64_t cap, uint64_t* truss_count,
+                                  kt_result* res);
+
 #ifdef __cplusplus
 }
 #endif

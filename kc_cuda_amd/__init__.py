@@ -24,7 +24,8 @@ from . import _lib
 from ._lib import NativeLibraryMissing  # noqa: F401
 
 __all__ = ["Context", "KtResult", "KtError", "Server", "ServeResult", "kmax_truss", "native_library",
-           "parse_tsv", "NativeLibraryMissing", "EXPORTS"]
+           "parse_tsv", "NativeLibraryMissing", "EXPORTS",
+           "solve_device_noalloc"]
 
 # Every symbol declared in include/ktruss.h and include/kc_gpu_c.h.
 EXPORTS = [
@@ -34,7 +35,7 @@ EXPORTS = [
     "kt_push_probe", "kt_push_stats_get",
     "kt_parse_tsv", "kt_parse_tsv_device", "kt_set_tuning", "kt_last_csr", "kt_last_stats", "kt_batch_solve",
     "kt_batch_autotune", "kt_serve_create", "kt_serve_destroy", "kt_serve_submit", "kt_serve_wait",
-    "kt_serve_counters_get", "kt_serve_footprint",
+    "kt_serve_counters_get", "kt_serve_footprint", "kt_solve_pairs_device_noalloc",
 ]
 
 
@@ -144,6 +145,8 @@ def native_library() -> ctypes.CDLL:
         "kt_serve_submit": [_V, _V, _U64, _U32, ctypes.c_int, ctypes.POINTER(_U64)],
         "kt_serve_wait": [_V, _U64, ctypes.POINTER(KtServeResult)],
         "kt_serve_counters_get": [_V, ctypes.POINTER(KtServeCounters)],
+        "kt_solve_pairs_device_noalloc": [ctypes.c_int, _V, _U64, _U32, _V, _U64, ctypes.POINTER(_U64),
+                                          ctypes.POINTER(KtResult)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -179,6 +182,20 @@ def parse_tsv(path: str | os.PathLike) -> tuple[np.ndarray, np.ndarray, int]:
     _check(lib.kt_parse_tsv(os.fsencode(str(path)), xs.ctypes.data, ys.ctypes.data, L,
                             ctypes.byref(lines), ctypes.byref(n)))
     return xs[:L], ys[:L], n.value
+
+
+def solve_device_noalloc(device: int, dpairs_ptr: int, m: int, n: int, dtruss_ptr: int = 0,
+                         cap: int = 0) -> tuple[KtResult, int]:
+    """The allocation-free solve (kt_solve_pairs_device_noalloc): m canonical
+    pairs at device address dpairs_ptr; when dtruss_ptr is non-zero, at most
+    cap kmax-truss pairs are written there (device memory). Returns the
+    result and the truss size."""
+    lib = native_library()
+    r = KtResult()
+    cnt = _U64()
+    _check(lib.kt_solve_pairs_device_noalloc(int(device), dpairs_ptr or None, int(m), int(n),
+                                             dtruss_ptr or None, int(cap), ctypes.byref(cnt), ctypes.byref(r)))
+    return r, int(cnt.value)
 
 
 @dataclass

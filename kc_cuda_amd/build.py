@@ -32,7 +32,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall",
              f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
 DEVICE_SOURCES = ["kt_kernels.hip", "kt_peel.hip", "kt_ingest.hip", "kt_batch_union.hip", "kt_solver.hip", "kt_stages.hip",
-                  "kt_compat.hip", "kt_batch_host.cpp", "kt_serve.cpp", "kt_capi.cpp", "kt_tsv.cpp"]
+                  "kt_compat.hip", "kt_noalloc.hip", "kt_batch_host.cpp", "kt_serve.cpp", "kt_capi.cpp", "kt_tsv.cpp"]
 
 
 def _run(cmd: list[str], quiet: bool) -> None:

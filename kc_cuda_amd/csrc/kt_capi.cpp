@@ -10,6 +10,7 @@
 
 #include "../../include/ktruss.h"
 #include "kt_batch_host.h"
+#include "kt_noalloc.h"
 #include "kt_serve.h"
 #include "kt_solver.h"
 #include "kt_tsv.h"
@@ -371,6 +372,16 @@ int kt_last_stats(kt_context* ctx, kt_stats* out) {
 int kt_last_csr(kt_context* ctx, uint64_t* off, uint32_t* adj) {
   if (!ctx) return set_err("kt_last_csr: NULL context");
   return guarded([&] { ctx->solver->csr_to_host(off, adj); });
+}
+
+int kt_solve_pairs_device_noalloc(int device, const uint32_t* d_pairs, uint64_t m, uint32_t n,
+                                  uint32_t* d_truss, uint64_t cap, uint64_t* truss_count,
+                                  kt_result* res) {
+  if (!res || (m && !d_pairs)) return set_err("kt_solve_pairs_device_noalloc: NULL argument");
+  return guarded([&] {
+    const uint64_t c = kt::noalloc_solve_pairs(device, d_pairs, m, n, d_truss, cap, res);
+    if (truss_count) *truss_count = c;
+  });
 }
 
 }  // extern "C"

@@ -8,10 +8,33 @@
 
 namespace kt {
 
+// A fixed block of device memory that buffers are carved from, in bump
+// order, instead of being allocated. reset() frees everything above `floor`
+// at once; nothing is ever returned to the HIP allocator.
+struct Arena {
+  uint8_t* base = nullptr;
+  size_t size = 0;
+  size_t floor = 0;  // bytes that survive reset()
+  size_t used = 0;
+  void reset() { used = floor; }
+};
+
+// While an ArenaScope is alive, DevBuf::ensure on the same host thread
+// carves from its arena and never calls hipMalloc (and DevBuf::release of a
+// carved buffer frees nothing).
+struct ArenaScope {
+  explicit ArenaScope(Arena* a);
+  ~ArenaScope();
+  ArenaScope(const ArenaScope&) = delete;
+  ArenaScope& operator=(const ArenaScope&) = delete;
+  Arena* prev;
+};
+
 // Grow-only device buffer.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool carved = false;  // p lies in an arena
   void ensure(size_t b);
   void release();
   template <typename T>
@@ -78,7 +101,15 @@ struct EdgeSet {
 class Solver {
  public:
   explicit Solver(int device);
+  // Arena mode: the control block and, inside an ArenaScope on `arena`,
+  // every device buffer come from the arena; readbacks go through host
+  // memory of the object itself. Nothing is allocated on the device.
+  Solver(int device, Arena* arena);
   ~Solver();
+  // Arena mode: drops every buffer (all carved) before the arena is reset.
+  void forget_buffers();
+  // Zeroes the control block (a fill kernel on the solver's stream).
+  void clear_control();
   // Device bytes this context holds now (its buffers keep their size
   // between solves, so after a solve this is at least that solve's peak).
   uint64_t device_bytes() const;
@@ -108,6 +139,8 @@ class Solver {
                           kt_result* res);
   // Copies the survivors of the last solve (original ids, u < v) to host.
   uint64_t truss_edges(uint32_t* host_pairs, uint64_t cap);
+  // The same into device memory (queued on the solver's stream, not waited for).
+  uint64_t truss_edges_to_device(uint32_t* dev_pairs, uint64_t cap);
   // Truss decomposition of the last input graph: (u, v, t) with u < v for
   // every edge of trussness t >= 3 (the others have 2), unordered; returns
   // the count, copies at most cap triples. Replaces the last solve's
@@ -223,10 +256,13 @@ class Solver {
   void sort_pairs_u64(const uint64_t* k_in, uint64_t* k_out, const uint32_t* v_in, uint32_t* v_out,
                       uint32_t count);
 
+  void init_common();
   int device_;
+  Arena* arena_ = nullptr;  // arena mode
   hipStream_t stream_ = nullptr;
   Ctl* ctl_ = nullptr;
   uint32_t* pinned_ = nullptr;
+  uint32_t host_rb_[64] = {};  // readback buffer in arena mode
   hipEvent_t ev_[10];
   int lanes_count_ = 0, lanes_peel_ = 0;
   bool force_edge_count_ = false;

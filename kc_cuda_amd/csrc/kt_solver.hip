@@ -11,8 +11,36 @@
 
 namespace kt {
 
+namespace {
+thread_local Arena* tl_arena = nullptr;
+}  // namespace
+
+ArenaScope::ArenaScope(Arena* a) : prev(tl_arena) { tl_arena = a; }
+ArenaScope::~ArenaScope() { tl_arena = prev; }
+
 void DevBuf::ensure(size_t b) {
   if (b <= bytes && p) return;
+  if (Arena* a = tl_arena) {
+    // carve: blocks are 256-byte aligned; the block on top of the arena
+    // grows in place, any other is left behind (reset() reclaims it)
+    const size_t want = (std::max<size_t>(b, 256) + 255) & ~size_t(255);
+    uint8_t* top = a->base + a->used;
+    if (p && carved && static_cast<uint8_t*>(p) + bytes == top && a->used - bytes + want <= a->size) {
+      a->used += want - bytes;
+      bytes = want;
+      return;
+    }
+    if (a->used + want > a->size)
+      fail("scratch exhausted: the solve needs more than " + std::to_string(a->size) +
+           " bytes of device scratch (" + std::to_string(a->used) + " in use, " + std::to_string(want) +
+           " more asked)");
+    p = top;
+    a->used += want;
+    bytes = want;
+    carved = true;
+    return;
+  }
+  if (carved) fail("a buffer carved from scratch was grown outside its scratch scope");
   // queued work may still read the old block
   if (p) KT_HIP(hipDeviceSynchronize());
   release();
@@ -23,9 +51,10 @@ void DevBuf::ensure(size_t b) {
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && !carved) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
+  carved = false;
 }
 
 namespace {
@@ -40,6 +69,39 @@ Solver::Solver(int device) : device_(device) {
   KT_HIP(hipMalloc(&ctl_, sizeof(Ctl)));
   KT_HIP(hipMemsetAsync(ctl_, 0, sizeof(Ctl), stream_));
   KT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 256, hipHostMallocDefault));
+  init_common();
+}
+
+Solver::Solver(int device, Arena* arena) : device_(device), arena_(arena) {
+  static_assert(sizeof(Ctl) <= 4096 && sizeof(Ctl) % 4 == 0, "control block exceeds its arena slot");
+  if (!arena || arena->size < 8192 || (reinterpret_cast<uintptr_t>(arena->base) & 255))
+    fail("arena mode: the arena must be 256-byte aligned and hold at least 8 KiB");
+  KT_HIP(hipSetDevice(device_));
+  KT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  ctl_ = reinterpret_cast<Ctl*>(arena->base);  // the arena's first 4 KiB, kept across resets
+  arena->floor = 4096;
+  arena->reset();
+  fill_u32(reinterpret_cast<uint32_t*>(ctl_), sizeof(Ctl) / 4, 0, stream_);
+  pinned_ = host_rb_;
+  init_common();
+}
+
+void Solver::forget_buffers() {
+  each_buffer([&](const DevBuf& b) {
+    if (b.p && !b.carved) fail("arena mode: a device buffer was allocated outside the arena");
+    DevBuf& w = const_cast<DevBuf&>(b);
+    w.p = nullptr;
+    w.bytes = 0;
+    w.carved = false;
+  });
+  have_truss_ = truss_ready_ = triples_ready_ = core_valid_ = false;
+  last_core_ = CoreView{};
+}
+
+void Solver::clear_control() { fill_u32(reinterpret_cast<uint32_t*>(ctl_), sizeof(Ctl) / 4, 0, stream_); }
+
+void Solver::init_common() {
+  static_assert(sizeof(host_rb_) >= 256, "readback buffer smaller than the pinned one");
   for (auto& e : ev_) KT_HIP(hipEventCreate(&e));
   for (auto& e : stage_ev_) KT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   KT_HIP(hipStreamSynchronize(stream_));
@@ -91,8 +153,10 @@ Solver::~Solver() {
   for (auto& e : probe_ev_) (void)hipEventDestroy(e);
   for (auto* p : stage_)
     if (p) (void)hipHostFree(p);
-  if (ctl_) (void)hipFree(ctl_);
-  if (pinned_) (void)hipHostFree(pinned_);
+  if (!arena_) {
+    if (ctl_) (void)hipFree(ctl_);
+    if (pinned_) (void)hipHostFree(pinned_);
+  }
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 

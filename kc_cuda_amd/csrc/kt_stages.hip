@@ -919,6 +919,15 @@ uint64_t Solver::trussness(uint32_t* host_triples, uint64_t cap) {
   return triples_count_;
 }
 
+uint64_t Solver::truss_edges_to_device(uint32_t* dev_pairs, uint64_t cap) {
+  KT_HIP(hipSetDevice(device_));
+  materialize_truss();
+  const uint64_t take = std::min<uint64_t>(truss_count_, cap);
+  if (take && dev_pairs)
+    KT_HIP(hipMemcpyAsync(dev_pairs, truss_out_.p, sizeof(uint32_t) * 2 * take, hipMemcpyDeviceToDevice, stream_));
+  return truss_count_;
+}
+
 uint64_t Solver::truss_edges(uint32_t* host_pairs, uint64_t cap) {
   KT_HIP(hipSetDevice(device_));
   materialize_truss();
