@@ -70,8 +70,13 @@ void core_assign_rank(const uint64_t* sorted_keys, const uint32_t* sorted_vals, 
 // (b') induced subgraph of the alive vertices
 void alive_list(const uint32_t* lvl, uint32_t n, uint32_t* list, uint32_t* new_id,
                 uint32_t* count, hipStream_t s);
+// sub_deg[i] = deg[list[i]] (skipped when deg is null), sub_orig[i] = its input id.
 void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_orig, uint32_t na,
                  uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s);
+// sub_deg[i] = neighbours of list[i] whose bit is set in the alive bitmap
+// abits (bit w % 32 of word w / 32).
+void list_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* list, uint32_t na,
+                        const uint32_t* abits, uint32_t* sub_deg, HeavyList hl, hipStream_t s);
 // Alive vertices of a level as a bitmap (bits: (n + 31) / 32 words) with the
 // popcount of every word in cnt (words + 1 entries, the last set to 0); an
 // exclusive scan of cnt gives pref, and ranked_list writes the alive vertices

@@ -593,8 +593,65 @@ __global__ void k_sub_degrees(const uint32_t* __restrict__ list, const uint32_t*
                               uint32_t* sub_orig) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
     const uint32_t v = list[i];
-    sub_deg[i] = deg[v];
+    if (deg) sub_deg[i] = deg[v];
     sub_orig[i] = to_orig ? to_orig[v] : v;
+  }
+}
+
+__device__ __forceinline__ uint32_t alive_bit(const uint32_t* __restrict__ bits, uint32_t w) {
+  return (bits[w >> 5] >> (w & 31)) & 1u;
+}
+
+// Degrees of the cut counted from the rows themselves: sub_deg[i] = the
+// neighbours of list[i] whose bit is set in the alive bitmap (n / 8 bytes,
+// so the lookups stay in L2 and the MALL). One wave per listed vertex; rows
+// longer than kHeavyRow go to the heavy list (by list index) for the sliced
+// kernel below.
+__global__ __launch_bounds__(kBlock) void k_list_alive_deg(const uint64_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ adj,
+                                                           const uint32_t* __restrict__ list, uint32_t na,
+                                                           const uint32_t* __restrict__ abits, uint32_t* sub_deg,
+                                                           uint32_t* heavy, uint32_t* heavy_count) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < na; i += waves) {
+    const uint32_t v = list[i];
+    const uint64_t b = off[v], e = off[v + 1];
+    if (e - b > kHeavyRow) {
+      if (lane == 0) {
+        sub_deg[i] = 0;
+        push_heavy(i, heavy, heavy_count);
+      }
+      continue;
+    }
+    uint32_t c = 0;
+    for (uint64_t j = b + lane; j < e; j += kWave) c += alive_bit(abits, adj[j]);
+    c = wave_sum(c);
+    if (lane == 0) sub_deg[i] = c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_list_alive_deg_heavy(const uint64_t* __restrict__ off,
+                                                                 const uint32_t* __restrict__ adj,
+                                                                 const uint32_t* __restrict__ list,
+                                                                 const uint32_t* __restrict__ abits,
+                                                                 const uint32_t* heavy,
+                                                                 const uint32_t* heavy_count,
+                                                                 const uint32_t* __restrict__ slices,
+                                                                 uint32_t* sub_deg) {
+  const uint32_t hc = *heavy_count;
+  if (hc == 0) return;
+  const uint32_t total = slices[hc];
+  for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+    const uint32_t h = slice_row(slices, hc, t);
+    const uint32_t i = heavy[h];
+    const uint32_t v = list[i];
+    const uint64_t b = off[v] + uint64_t(t - slices[h]) * kSlice;
+    const uint64_t e = min(off[v + 1], b + kSlice);
+    uint32_t c = 0;
+    for (uint64_t j = b + threadIdx.x; j < e; j += blockDim.x) c += alive_bit(abits, adj[j]);
+    c = wave_sum(c);
+    if (lane64() == 0 && c) atomicAdd(&sub_deg[i], c);
   }
 }
 
@@ -1335,6 +1392,16 @@ void alive_list(const uint32_t* lvl, uint32_t n, uint32_t* list, uint32_t* new_i
 void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_orig, uint32_t na,
                  uint32_t* sub_deg, uint32_t* sub_orig, hipStream_t s) {
   if (na) k_sub_degrees<<<grid_for(na), kBlock, 0, s>>>(list, deg, to_orig, na, sub_deg, sub_orig);
+}
+void list_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* list, uint32_t na,
+                        const uint32_t* abits, uint32_t* sub_deg, HeavyList hl, hipStream_t s) {
+  if (!na) return;
+  fill_u32(hl.count, 1, 0, s);
+  k_list_alive_deg<<<std::min(grid_for(uint64_t(na) * kWave), 8192), kBlock, 0, s>>>(off, adj, list, na, abits,
+                                                                                     sub_deg, hl.items, hl.count);
+  k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, list, hl.slices, nullptr);
+  k_list_alive_deg_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, list, abits, hl.items, hl.count, hl.slices,
+                                                       sub_deg);
 }
 void alive_words(const uint32_t* lvl, uint32_t n, uint32_t* bits, uint32_t* cnt, hipStream_t s) {
   const uint32_t words = (n + 31) / 32;

@@ -1,5 +1,6 @@
 // Host orchestration of the device-resident k-max-truss solve.
 #pragma once
+#include <chrono>
 #include <memory>
 #include <vector>
 
@@ -191,7 +192,18 @@ class Solver {
   uint32_t max_core(bool allow_cut = true, bool fresh = true);
   bool filtered_start(LevelGraph** gp, uint32_t* k_out, uint64_t* alive_out);
   LevelGraph& level0();
-  LevelGraph& cut_level(LevelGraph& parent, uint32_t k_enter, uint32_t alive);
+  // recount: the alive vertices' degrees in `parent` are stale (a degree
+  // filter stamped vertices without decrementing their neighbours), so the
+  // cut counts them from the rows.
+  LevelGraph& cut_level(LevelGraph& parent, uint32_t k_enter, uint32_t alive, bool recount = false);
+  // The k-core of the input by a degree filter: vertices of degree < k are
+  // stamped removed at k on level 0, the rest is cut (degrees recounted) and
+  // peeled at k. Returns the k-core's vertex count; depth_ = 2 when it is
+  // non-empty.
+  uint64_t filtered_core(uint32_t k);
+  // Vertices of input degree >= k (exact below kDegExact, else a lower
+  // bound), from the degree histogram of the current input; ~0 when none.
+  uint64_t degree_at_least(uint32_t k) const;
   void core_at(uint32_t k);  // leaves the k-core alive on the deepest usable level
   void extract_core(LevelGraph& g, CoreView* cv);
   void sort_oriented_rows(uint32_t nc, uint32_t mc, uint32_t max_row);
@@ -269,10 +281,14 @@ class Solver {
   bool force_atomic_build_ = false;
   int core_mode_ = 0;
   DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_, dhist_;
+  std::vector<uint64_t> deg_at_least_;  // input degree histogram, suffix sums per bin (empty: none)
   DevBuf bk_state_, bk_sup_, bk_bits_, bk_del_;  // threshold-probe backups
   uint64_t eidm_budget_ = uint64_t(4) << 30;  // bytes for the edge-id matrix
   double dense_tri_cost_ = 4.0, sparse_drop_ratio_ = 3.0, default_drop_ratio_ = 3.0;
   bool trace_ = false;  // KT_TRACE=1: per-round frontier sizes on stderr
+  // trace only: milliseconds since the previous call, after the stream drains
+  double trace_ms();
+  std::chrono::steady_clock::time_point trace_t0_{};
   bool gallop_ = true;
   uint32_t count_kind_ = 0, count_words_ = 0;  // the last timed support count
   uint32_t walk_levels_ = 16;

@@ -98,6 +98,14 @@ void Solver::forget_buffers() {
   last_core_ = CoreView{};
 }
 
+double Solver::trace_ms() {
+  KT_HIP(hipStreamSynchronize(stream_));
+  const auto now = std::chrono::steady_clock::now();
+  const double ms = std::chrono::duration<double, std::milli>(now - trace_t0_).count();
+  trace_t0_ = now;
+  return ms;
+}
+
 void Solver::clear_control() { fill_u32(reinterpret_cast<uint32_t*>(ctl_), sizeof(Ctl) / 4, 0, stream_); }
 
 void Solver::init_common() {
@@ -247,6 +255,7 @@ bool Solver::build_partitioned(int mode, const uint32_t* a, const uint32_t* b, u
 }
 
 void Solver::new_input() {
+  deg_at_least_.clear();
   have_truss_ = false;
   truss_ready_ = false;
   triples_ready_ = false;

@@ -83,7 +83,7 @@ uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
 }
 
 // Induced subgraph of the alive vertices of `p` as the next level.
-LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
+LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive, bool recount) {
   if (levels_.size() <= depth_) levels_.emplace_back(new LevelGraph);
   LevelGraph& s = *levels_[depth_];
   // alive bitmap | word popcounts | their exclusive scan, then the ranked list
@@ -110,7 +110,8 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
   s.k_enter = k_enter;
   s.ok.assign(1, k_enter);
   fill_u32(s.deg + na, 1, 0, stream_);
-  sub_degrees(list_.as<uint32_t>(), p.deg, p.to_orig, na, s.deg, s.to_orig, stream_);
+  sub_degrees(list_.as<uint32_t>(), recount ? nullptr : p.deg, p.to_orig, na, s.deg, s.to_orig, stream_);
+  if (recount) list_alive_degrees(p.off, p.adj, list_.as<uint32_t>(), na, abits, s.deg, heavy(), stream_);
   exclusive_scan_u32_to_u64(s.deg, s.off, na + 1);
   s.slots = read_u64(s.off + na);
   s.b_adj.ensure(sizeof(uint32_t) * std::max<uint64_t>(s.slots, 1));
@@ -124,17 +125,45 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive) {
   return s;
 }
 
-// Start of the max-core search from a high threshold. The k-core lies
-// inside the subgraph induced by the vertices of degree >= k (and is that
-// subgraph's k-core), so a threshold below the max core is reached by one
-// filter of the vertices, one recount of the kept rows and a cut, and the
-// peel then runs on the small induced subgraph instead of cascading through
-// the whole graph from threshold 1. The threshold is an eighth of a lower
+uint64_t Solver::degree_at_least(uint32_t k) const {
+  if (deg_at_least_.empty()) return ~uint64_t(0);
+  if (k < kDegExact) return deg_at_least_[k];
+  uint32_t b = kDegExact + (31u - __builtin_clz(k)) - kDegExactLog2;
+  if ((uint64_t(1) << (b - kDegExact + kDegExactLog2)) < k) ++b;
+  return deg_at_least_[std::min<uint32_t>(b, kDegBins)];
+}
+
+// The k-core lies inside the subgraph induced by the vertices of degree >= k
+// (and is that subgraph's k-core), so it is reached by one filter of the
+// vertices, a cut that counts the kept vertices' degrees from their rows,
+// and a peel of the small induced subgraph, instead of a cascade through the
+// whole graph.
+uint64_t Solver::filtered_core(uint32_t k) {
+  depth_ = 1;
+  LevelGraph& g0 = *levels_[0];
+  const uint32_t n = g0.n;
+  degrees_from_offsets(g0.off, n, g0.deg, stream_);
+  zero_u32(&ctl_->nalive);
+  core_filter(g0.deg, n, k, g0.lvl, &ctl_->nalive, stream_);
+  const uint32_t kept = read_u32(&ctl_->nalive);
+  if (kept == 0) return 0;  // every vertex is stamped removed at k
+  LevelGraph& s = cut_level(g0, k, kept, true);
+  const uint64_t removed = peel_vertices(s, k);
+  if (removed >= kept) {  // the k-core is empty: level 0 says so too
+    depth_ = 1;
+    fill_u32(g0.lvl, n, k, stream_);
+    return 0;
+  }
+  return kept - removed;
+}
+
+// Start of the max-core search from a high threshold: an eighth of a lower
 // bound of the degree h-index H (at least H vertices have degree >= H; the
-// max core is at most H); when its core is empty the threshold drops by 4x,
-// and when the filter would keep more than a quarter of the vertices the
-// plain search from 1 runs instead. On success *gp is the cut level with the
-// kf-core alive, *k_out = kf and *alive_out its vertex count.
+// max core is at most H), reached by filtered_core; when its core is empty
+// the threshold drops by 4x, and when the filter would keep more than a
+// quarter of the vertices the plain search from 1 runs instead. On success
+// *gp is the cut level with the kf-core alive, *k_out = kf and *alive_out its
+// vertex count. The degree histogram stays for later filtered cuts.
 bool Solver::filtered_start(LevelGraph** gp, uint32_t* k_out, uint64_t* alive_out) {
   LevelGraph& g0 = **gp;
   const uint32_t n = g0.n;
@@ -144,44 +173,31 @@ bool Solver::filtered_start(LevelGraph** gp, uint32_t* k_out, uint64_t* alive_ou
   std::vector<uint32_t> h(kDegBins);
   KT_HIP(hipMemcpyAsync(h.data(), dhist_.p, sizeof(uint32_t) * kDegBins, hipMemcpyDeviceToHost, stream_));
   KT_HIP(hipStreamSynchronize(stream_));
-  // at_least[b]: vertices whose degree is at least the lower edge of bin b
-  std::vector<uint64_t> at_least(kDegBins + 1, 0);
+  // deg_at_least_[b]: vertices whose degree is at least the lower edge of bin b
+  deg_at_least_.assign(kDegBins + 1, 0);
   uint64_t hidx = 0;
   for (int b = int(kDegBins) - 1; b >= 0; --b) {
-    at_least[b] = at_least[b + 1] + h[b];
+    deg_at_least_[b] = deg_at_least_[b + 1] + h[b];
     const uint64_t lo = uint32_t(b) < kDegExact ? uint64_t(b) : uint64_t(1) << (b - kDegExact + kDegExactLog2);
-    hidx = std::max(hidx, std::min(at_least[b], lo));
+    hidx = std::max(hidx, std::min(deg_at_least_[b], lo));
   }
-  auto kept_at_least = [&](uint32_t k) {  // vertices of degree >= k (exact below kDegExact, else a lower bound)
-    if (k < kDegExact) return at_least[k];
-    uint32_t b = kDegExact + (31u - __builtin_clz(k)) - kDegExactLog2;
-    if ((uint64_t(1) << (b - kDegExact + kDegExactLog2)) < k) ++b;
-    return at_least[std::min<uint32_t>(b, kDegBins)];
-  };
   for (uint32_t kf = uint32_t(hidx / 8); kf >= 3; kf /= 4) {
-    if (kept_at_least(kf) * 4 > n) return false;
-    zero_u32(&ctl_->nalive);
-    core_filter(g0.deg, n, kf, g0.lvl, &ctl_->nalive, stream_);
-    const uint32_t kept = read_u32(&ctl_->nalive);
-    if (kept > 0) {
-      core_alive_degrees(g0.off, g0.adj, g0.lvl, n, g0.deg, heavy(), stream_);
-      LevelGraph& s = cut_level(g0, kf, kept);
-      const uint64_t removed = peel_vertices(s, kf);
-      if (trace_)
-        std::fprintf(stderr, "[kt] max-core filter k=%u (h-index >= %llu): %u kept, %llu in the core\n", kf,
-                     (unsigned long long)hidx, kept, (unsigned long long)(kept - std::min<uint64_t>(removed, kept)));
-      if (removed < kept) {
-        *gp = &s;
-        *k_out = kf;
-        *alive_out = kept - removed;
-        return true;
-      }
+    if (degree_at_least(kf) > n / 4) break;
+    const uint64_t alive = filtered_core(kf);
+    if (trace_)
+      std::fprintf(stderr, "[kt] max-core filter k=%u (h-index >= %llu): %llu in the core\n", kf,
+                   (unsigned long long)hidx, (unsigned long long)alive);
+    if (alive) {
+      *gp = levels_[1].get();
+      *k_out = kf;
+      *alive_out = alive;
+      return true;
     }
-    // the kf-core is empty: back to the whole graph
-    depth_ = 1;
-    degrees_from_offsets(g0.off, n, g0.deg, stream_);
-    fill_u32(g0.lvl, n, kNone, stream_);
   }
+  // back to the whole graph
+  depth_ = 1;
+  degrees_from_offsets(g0.off, n, g0.deg, stream_);
+  fill_u32(g0.lvl, n, kNone, stream_);
   return false;
 }
 
@@ -240,6 +256,14 @@ uint32_t Solver::max_core(bool allow_cut, bool fresh) {
 void Solver::core_at(uint32_t k) {
   size_t d = depth_ - 1;
   while (d > 0 && levels_[d]->k_enter > k) --d;
+  if (d == 0 && k >= 3 && levels_[0]->n >= 4 * kMinCut && degree_at_least(k) <= levels_[0]->n / 4) {
+    // no cut level holds the k-core, but few vertices have degree >= k:
+    // filter and cut again rather than re-peel the whole graph
+    const uint64_t alive = filtered_core(k);
+    if (trace_) std::fprintf(stderr, "[kt] core_at k=%u by degree filter: %llu in the core\n", k,
+                             (unsigned long long)alive);
+    return;
+  }
   depth_ = d + 1;
   LevelGraph& g = *levels_[d];
   uint32_t kj = std::max<uint32_t>(g.k_enter, 1);
@@ -646,7 +670,11 @@ TrussOutcome Solver::truss_gallop(CoreView* cv, int64_t empty_from) {
   SearchRule walk{};
   walk.max_levels = walk_levels_;
   uint32_t r = 1;
+  if (trace_) trace_ms();
   TrussOutcome lv = run_levels(cv, 1, cv->mc, &r, walk, true);
+  if (trace_)
+    std::fprintf(stderr, "[kt] gallop walk of %u levels: lo=%d alive=%llu rounds=%llu (%.2f ms)\n", lv.levels,
+                 int(lv.kmax) - 2, (unsigned long long)lv.count, (unsigned long long)lv.rounds, trace_ms());
   if (!lv.cut_short) return lv;  // the walk finished the search
   out.rounds += lv.rounds;
   out.levels += lv.levels;
@@ -699,8 +727,9 @@ TrussOutcome Solver::truss_gallop(CoreView* cv, int64_t empty_from) {
         out.floor_thr = w.floor_thr;
       }
       if (trace_)
-        std::fprintf(stderr, "[kt] gallop walk from t=%lld -> kmax=%u count=%llu\n", (long long)(lo + 1), out.kmax,
-                     (unsigned long long)out.count);
+        std::fprintf(stderr, "[kt] gallop walk from t=%lld -> kmax=%u count=%llu levels=%u rounds=%llu (%.2f ms)\n",
+                     (long long)(lo + 1), out.kmax, (unsigned long long)out.count, w.levels,
+                     (unsigned long long)w.rounds, trace_ms());
       break;
     }
     // gallop up by doubling steps, never past the bound; bisect once a
@@ -725,8 +754,9 @@ TrussOutcome Solver::truss_gallop(CoreView* cv, int64_t empty_from) {
       probed_empty = true;
     }
     if (trace_)
-      std::fprintf(stderr, "[kt] gallop probe t=%lld -> %s lo=%lld hi=%lld alive=%llu\n", (long long)t,
-                   p.alive ? "kept" : "empty", (long long)lo, (long long)hi, (unsigned long long)alive);
+      std::fprintf(stderr, "[kt] gallop probe t=%lld -> %s lo=%lld hi=%lld alive=%llu rounds=%llu recounts=%u (%.2f ms)\n",
+                   (long long)t, p.alive ? "kept" : "empty", (long long)lo, (long long)hi, (unsigned long long)alive,
+                   (unsigned long long)p.rounds, p.recounts, trace_ms());
   }
   return out;
 }
