@@ -16,13 +16,16 @@ rmat22` is RMAT scale 22, edge factor 16, (.57,.19,.19,.05), seed 1.
 
 Prints one JSON line on stdout (everything else goes to stderr):
   value        edges/s = m / wall time per solve
-  roofline     the kernel with the largest share of the solve's device time
-               (KERNEL_ROOFLINE): its algorithmic bytes per launch over its
-               average launch duration, measured with HIP events stamped at
-               each launch's start and end on the solver's stream in an extra
-               untimed solve, against 8 TB/s HBM3E; `traffic` is the
-               rocprofv3-measured HBM bytes per launch when a summary for this
-               kernel and workload is committed under profiles/
+  roofline     of the vertex-peel kernels (k_core_push, k_core_cascade) and
+               the bit-matrix edge-peel round (k_dense_peel_round), the one with more device
+               time per solve: its algorithmic bytes per launch (KERNEL_MODELS,
+               the path's per-unit byte figures times the units the launches
+               process) over its average launch duration, measured with HIP
+               events stamped at each launch's start and end on the solver's
+               stream in an extra untimed solve, against 8 TB/s HBM3E;
+               `traffic` is the rocprofv3-measured HBM bytes per launch when a
+               summary for this kernel and workload is committed under
+               profiles/; the other kernel is config.second_kernel_roofline
   config.solve_bytes
                the whole solve's algorithmic bytes (the path's byte model)
                over the wall time, and the rocprofv3-measured HBM bytes of one
@@ -170,32 +173,39 @@ def _traffic(kernel: str, workload: str) -> dict | None:
         return None
 
 
-def push_roofline(ctx, d, m: int, n: int, workload: str) -> dict:
-    """The vertex-peel push k_core_push: one untimed solve with HIP events
-    stamped at every launch's start and end on the solver's stream, and one
-    that counts what the launches walk (its atomics stay out of the timing).
-    Algorithmic bytes per launch: per row slot walked the adjacency entry
-    (4 B), the neighbour's lvl (4 B) and its degree decrement (4 B); per
-    vertex taken its queue entry (4 B) and two offsets (16 B)."""
-    ctx.push_probe(1)
+KERNEL_MODELS = {
+    # kernel probe -> (kernel name, bytes from the counters, per-unit figure)
+    # levels of up to 2^24 row slots are peeled by k_core_cascade, larger ones by k_core_push rounds
+    "vertex_peel": ("k_core_push+k_core_cascade", lambda slots, items: 4 * slots + 12 * items,
+                    "4 B per row slot walked + 12 B per vertex taken (the vertex-peel term 8m + 12n + 8)"),
+    "dense_peel": ("k_dense_peel_round", lambda slots, items: 4 * slots,
+                   "4 B x (d+(u) + d+(v)) per frontier edge walked (the peel term 4 W(C+ minus T+))"),
+}
+
+
+def kernel_roofline(ctx, d, m: int, n: int, workload: str, probe: str) -> dict:
+    """One untimed solve with HIP events stamped at every launch's start and
+    end of the probed kernel on the solver's stream, and one that counts the
+    units the launches process (their atomics stay out of the timing)."""
+    name, model, per_unit = KERNEL_MODELS[probe]
+    ctx.kernel_probe(probe, 1)
     ctx.solve_device(d, m, n)
     ps = ctx.push_stats()
-    ctx.push_probe(2)
+    ctx.kernel_probe(probe, 2)
     ctx.solve_device(d, m, n)
     counted = ctx.push_stats()
-    ctx.push_probe(0)
+    ctx.kernel_probe(probe, 0)
     launches = max(int(ps["launches"]), 1)
-    per_launch = (12 * counted["slots"] + 20 * counted["vertices"]) / launches
+    per_launch = model(counted["slots"], counted["vertices"]) / launches
     us = ps["ms"] * 1e3 / launches
     achieved = per_launch / (us * 1e-6) / 1e9 if us > 0 else 0.0
-    log(f"[bench] k_core_push: {ps['launches']} launches, {us:.1f} us avg, {counted['slots']} slots, "
-        f"{counted['vertices']} vertices -> {achieved:.1f} GB/s")
-    t = _traffic("k_core_push", workload)
-    return {"kernel": "k_core_push", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": frac_of(achieved, "k_core_push"),
-            "traffic": t.get("hbm_bytes_per_launch") if t else None,
+    log(f"[bench] {name}: {ps['launches']} launches, {us:.1f} us avg ({ps['ms']:.2f} ms in all), "
+        f"units {counted['slots']} / {counted['vertices']} -> {achieved:.1f} GB/s")
+    t = _traffic(name, workload)
+    return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": frac_of(achieved, name), "traffic": t.get("hbm_bytes_per_launch") if t else None,
             "algorithmic_bytes": per_launch, "launches": int(ps["launches"]), "avg_launch_us": us,
-            "per_unit": "12 B per row slot walked + 20 B per vertex taken"}
+            "total_ms": ps["ms"], "per_unit": per_unit}
 
 
 def plain_line(args) -> dict:
@@ -252,7 +262,12 @@ def plain_line(args) -> dict:
     if ct and count_s > 0:
         count["measured_hbm_bytes"] = ct.get("hbm_bytes_per_launch")
         count["measured_hbm_gbs"] = count["measured_hbm_bytes"] / count_s / 1e9
-    roof = push_roofline(ctx, d, m, n, args.workload)
+    roof, other = None, None
+    if not args.no_probe:
+        # the two largest kernels of a solve; the one with more device time is the line's roofline
+        rv = kernel_roofline(ctx, d, m, n, args.workload, "vertex_peel")
+        rd = kernel_roofline(ctx, d, m, n, args.workload, "dense_peel") if st.count_kernel == 2 else None
+        roof, other = (rd, rv) if rd and rd["total_ms"] > rv["total_ms"] else (rv, rd)
     if args.calibrate_copy_mib > 0:
         nb = args.calibrate_copy_mib << 20
         a, b = ctx.alloc(nb), ctx.alloc(nb)
@@ -273,7 +288,8 @@ def plain_line(args) -> dict:
                    "k_hat": r.k_hat, "max_core": r.max_core, "core_k": r.core_k,
                    "n_core": st.n_core, "m_core": int(st.m_core), "w_core": int(st.w_core),
                    "w_peeled": int(st.w_peeled), "stages_ms": stages, "compactions": r.compactions,
-                   "recounts": r.recounts, "solve_bytes": solve, "count_kernel": count},
+                   "recounts": r.recounts, "solve_bytes": solve, "count_kernel": count,
+                   "second_kernel_roofline": other},
         "roofline": roof,
         "cpu_baseline": base,
     }
@@ -499,6 +515,8 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=None, help="batch: timed batch solves")
     ap.add_argument("--batch-warmup", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the roofline kernel's extra probe solves (under a profiler)")
     ap.add_argument("--cpu-budget", type=float, default=120.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after timing, one device-to-device copy of this size (HBM counter "

@@ -70,6 +70,8 @@ typedef struct kt_stats {
   uint64_t truss_edges;
   uint32_t count_kernel;  /* pass-2 support count: 0 LDS bitmap, 1 per edge, 2 bit matrix */
   uint32_t core_words;    /* bit-matrix row words (count_kernel 2) */
+  uint32_t vertex_rounds; /* the last vertex peel: 0 one cascade launch, 1 synchronous rounds */
+  uint32_t pad;
 } kt_stats;
 
 /* Library and device */
@@ -133,14 +135,20 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
  * bit-matrix core representation (default: by density), KT_FORCE_LEVEL_WALK
  * the level-by-level first pass on bit-matrix cores (default: 16 levels are
  * walked, then thresholds are probed), KT_FORCE_PROBES threshold probes right
- * after the first level. Every variant computes the same result. The
- * sparse-peel drop ratio returns to its default (KT_SPARSE_DROP_RATIO or 3). */
+ * after the first level, KT_FORCE_VERTEX_ROUNDS / KT_FORCE_VERTEX_CASCADE
+ * the vertex peel by synchronous rounds / by one cascade launch per
+ * threshold at every level (default: the cascade on levels of up to 2^24
+ * row slots, rounds above). Every variant computes the same result. The
+ * sparse-peel drop ratio returns to its default (KT_SPARSE_DROP_RATIO or
+ * 3). */
 #define KT_FORCE_EDGE_COUNT 1
 #define KT_FORCE_ATOMIC_BUILD 2
 #define KT_FORCE_SPARSE 4
 #define KT_FORCE_DENSE 8
 #define KT_FORCE_LEVEL_WALK 16
 #define KT_FORCE_PROBES 32
+#define KT_FORCE_VERTEX_ROUNDS 64
+#define KT_FORCE_VERTEX_CASCADE 128
 int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
 
 /* Batched solve of `count` independent graphs (no counterpart in the
@@ -246,6 +254,15 @@ typedef struct kt_push_stats {
 } kt_push_stats;
 int kt_push_probe(kt_context* ctx, int mode);
 int kt_push_stats_get(kt_context* ctx, kt_push_stats* out);
+/* The same measurement for another kernel: KT_PROBE_VERTEX_PEEL is what
+ * kt_push_probe measures (the vertex-peel cascade launch, or the push of
+ * the round-based peel); KT_PROBE_DENSE_PEEL is the bit-matrix edge-peel
+ * round (k_dense_peel_round), whose counters in kt_push_stats are slots =
+ * the frontier edges' oriented out-degrees d+(u) + d+(v) summed, vertices =
+ * the frontier edges walked. */
+#define KT_PROBE_VERTEX_PEEL 0
+#define KT_PROBE_DENSE_PEEL 1
+int kt_kernel_probe(kt_context* ctx, int kernel, int mode);
 
 /* Byte-model terms of the last solve (one extra device pass, untimed). */
 int kt_last_stats(kt_context* ctx, kt_stats* out);

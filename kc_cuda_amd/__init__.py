@@ -32,7 +32,7 @@ EXPORTS = [
     "kt_last_error", "kt_device_count", "kt_create", "kt_destroy", "kt_malloc", "kt_free",
     "kt_memcpy_h2d", "kt_memcpy_d2h", "kt_memcpy_d2d", "kt_synchronize", "kt_solve_pairs_device",
     "kt_solve_pairs_host", "kt_solve_lines_host", "kt_solve_tsv", "kt_truss_edges", "kt_trussness",
-    "kt_push_probe", "kt_push_stats_get",
+    "kt_push_probe", "kt_push_stats_get", "kt_kernel_probe",
     "kt_parse_tsv", "kt_parse_tsv_device", "kt_set_tuning", "kt_last_csr", "kt_last_stats", "kt_batch_solve",
     "kt_batch_autotune", "kt_serve_create", "kt_serve_destroy", "kt_serve_submit", "kt_serve_wait",
     "kt_serve_counters_get", "kt_serve_footprint", "kt_solve_pairs_device_noalloc",
@@ -64,7 +64,8 @@ class KtStats(ctypes.Structure):
     _fields_ = [("n_core", ctypes.c_uint32), ("max_out_degree", ctypes.c_uint32),
                 ("m_core", ctypes.c_uint64), ("w_core", ctypes.c_uint64),
                 ("w_peeled", ctypes.c_uint64), ("truss_edges", ctypes.c_uint64),
-                ("count_kernel", ctypes.c_uint32), ("core_words", ctypes.c_uint32)]
+                ("count_kernel", ctypes.c_uint32), ("core_words", ctypes.c_uint32),
+                ("vertex_rounds", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -138,6 +139,7 @@ def native_library() -> ctypes.CDLL:
         "kt_trussness": [_V, _V, _U64, ctypes.POINTER(_U64)],
         "kt_push_probe": [_V, ctypes.c_int],
         "kt_push_stats_get": [_V, ctypes.POINTER(KtPushStats)],
+        "kt_kernel_probe": [_V, ctypes.c_int, ctypes.c_int],
         "kt_batch_solve": [_V, _V, _V, _V, _U32, _V, _V, _V, _V, ctypes.c_int],
         "kt_batch_autotune": [_V, _V, _V, _V, _U32, _U32, _V, _V, _V, ctypes.POINTER(KtBatchTuning)],
         "kt_serve_create": [ctypes.c_int, ctypes.c_int, _U64, ctypes.POINTER(_V)],
@@ -249,13 +251,18 @@ class Context:
 
     # -- solves -------------------------------------------------------------
     def tune(self, count_lanes: int = 0, peel_lanes: int = 0, force_edge_count: bool = False,
-             force_atomic_build: bool = False, core: str = "auto", pass1: str = "auto"):
+             force_atomic_build: bool = False, core: str = "auto", pass1: str = "auto",
+             vertex: str = "auto"):
         """core: "auto", "sparse" (sorted rows) or "dense" (bit-matrix rows);
         pass1 on bit-matrix cores: "auto" (walk 16 levels, then probe
-        thresholds), "walk" (every level) or "probe" (probe after level 1)."""
+        thresholds), "walk" (every level) or "probe" (probe after level 1);
+        vertex: the vertex peel as one "cascade" launch per threshold, as
+        synchronous "rounds", or "auto" (the cascade on levels of up to
+        2^24 row slots)."""
         flags = (1 if force_edge_count else 0) | (2 if force_atomic_build else 0)
         flags |= {"auto": 0, "sparse": 4, "dense": 8}[core]
         flags |= {"auto": 0, "walk": 16, "probe": 32}[pass1]
+        flags |= {"auto": 0, "rounds": 64, "cascade": 128}[vertex]
         _check(self._lib.kt_set_tuning(self._h, count_lanes, peel_lanes, flags))
 
     def solve_pairs(self, pairs: np.ndarray, n: int) -> KtResult:
@@ -313,6 +320,12 @@ class Context:
         """Measurement of the vertex-peel push kernel: 1 stamps events at
         each launch's start and end, 2 counts its slots and vertices, 0 off."""
         _check(self._lib.kt_push_probe(self._h, int(mode)))
+
+    def kernel_probe(self, kernel: str, mode: int) -> None:
+        """Measurement of "vertex_peel" (as push_probe) or "dense_peel" (the
+        bit-matrix edge-peel round; push_stats then gives slots = the
+        frontier edges' d+(u) + d+(v) summed, vertices = frontier edges)."""
+        _check(self._lib.kt_kernel_probe(self._h, {"vertex_peel": 0, "dense_peel": 1}[kernel], int(mode)))
 
     def push_stats(self) -> dict:
         s = KtPushStats()

@@ -241,6 +241,11 @@ int kt_push_probe(kt_context* ctx, int on) {
   return guarded([&] { ctx->solver->set_push_probe(on); });
 }
 
+int kt_kernel_probe(kt_context* ctx, int kernel, int mode) {
+  if (!ctx) return set_err("kt_kernel_probe: NULL context");
+  return guarded([&] { ctx->solver->set_probe(kernel, mode); });
+}
+
 int kt_push_stats_get(kt_context* ctx, kt_push_stats* out) {
   if (!ctx || !out) return set_err("kt_push_stats_get: NULL argument");
   return guarded([&] { ctx->solver->push_stats(&out->launches, &out->ms, &out->slots, &out->vertices); });
@@ -263,6 +268,7 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
   ctx->solver->set_force_atomic_build((flags & KT_FORCE_ATOMIC_BUILD) != 0);
   ctx->solver->set_core_mode((flags & KT_FORCE_SPARSE) ? 1 : (flags & KT_FORCE_DENSE) ? 2 : 0);
   ctx->solver->set_gallop((flags & KT_FORCE_LEVEL_WALK) == 0, (flags & KT_FORCE_PROBES) ? 1u : 16u);
+  ctx->solver->set_vertex_mode((flags & KT_FORCE_VERTEX_ROUNDS) ? 1 : (flags & KT_FORCE_VERTEX_CASCADE) ? 2 : 0);
   ctx->solver->reset_drop_ratio();
   g_err.clear();
   return 0;

@@ -68,7 +68,19 @@ struct Ctl {
   unsigned long long early_ans;
   uint32_t exact_early;  // ... at kmax early_k (or any kmax when set)
   uint32_t max_levels;   // stop after this many completed levels (0: no limit)
+  // Vertex-peel cascade queue (one launch per threshold): items reserved,
+  // claimed and finished, chunk items added for long rows, and an error
+  // flag that makes every wave leave.
+  uint32_t vq_reserve;
+  uint32_t vq_head;
+  uint32_t vq_done;
+  uint32_t vq_chunks;
+  uint32_t vq_err;
 };
+
+// ctl->vq_err values
+constexpr uint32_t kVqOverflow = 1;  // an item past the queue's capacity
+constexpr uint32_t kVqStall = 2;     // a wave waited past its spin limit
 
 // ctl->err bit: a level ended with live edges but no live support (or the
 // reverse); the edge-peel bookkeeping is inconsistent
@@ -82,6 +94,15 @@ constexpr uint32_t kRoundCompact = 4;  // compact the edge set first (unstamped)
 constexpr uint32_t kRoundMinSup = 5;   // level done: reduce the live supports
 constexpr uint32_t kRoundSweep = 6;    // next level: collect its frontier
 constexpr uint32_t kRoundDone = 7;     // search finished (early exit, no edges)
+
+// Measurement of a round kernel (all optional): ev holds a start and an end
+// event per launch, stamped at the kernel's own start and end; stats gains
+// [0] the oriented out-degrees d+(u) + d+(v) of the frontier edges walked
+// and [1] their count.
+struct LaunchProbe {
+  hipEvent_t* ev = nullptr;
+  unsigned long long* stats = nullptr;
+};
 
 // Host-set parameters of the device round decision (drop or walk a frontier).
 struct RoundRule {
@@ -112,6 +133,7 @@ struct PeelArgs {
   int32_t* sup;
   uint32_t mc, nc, ws;
   const uint32_t *off_u, *adj_u, *eid_u;  // undirected view (sorted rows)
+  const uint32_t* off_d;                  // oriented rows (out-degrees, for measurement)
   uint64_t *bits, *fbits;                 // dense views: live / frontier bits
   const uint64_t* bits0;
   const uint32_t *pref0, *eidm;

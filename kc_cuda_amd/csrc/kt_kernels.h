@@ -47,6 +47,16 @@ void core_round_start(Ctl* ctl, hipStream_t s);
 void core_round(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t* q, Ctl* ctl, uint32_t k,
                 uint32_t* lvl, uint32_t* deg, HeavyList hl, unsigned long long* push_stats, hipEvent_t ev0,
                 hipEvent_t ev1, hipStream_t s);
+// The same peel at one threshold k as a single cascade launch (a work queue
+// of vertices, each queued once: by the initial sweep of degree < k or when a
+// decrement takes its degree from k to k - 1). q holds cap 8-byte items
+// (cascade_capacity of the level); afterwards ctl->vq_reserve - vq_chunks
+// vertices were removed (lvl = k), and ctl->vq_err is non-zero if the queue
+// overflowed or a wave gave up waiting. Measurement as for core_round.
+uint64_t cascade_capacity(uint32_t n, uint64_t slots);
+void core_cascade(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t k, uint32_t* lvl, uint32_t* deg,
+                  unsigned long long* q, uint32_t cap, Ctl* ctl, unsigned long long* stats, hipEvent_t ev0,
+                  hipEvent_t ev1, hipStream_t s);
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl,
                         uint32_t n, uint32_t* deg, HeavyList hl, hipStream_t s);
 // lvl[v] = (lvl[v] > k) ? kNone : (lvl[v] == kNone ? kNone : k) -- used to
@@ -76,7 +86,11 @@ void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_o
 // sub_deg[i] = neighbours of list[i] whose bit is set in the alive bitmap
 // abits (bit w % 32 of word w / 32).
 void list_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* list, uint32_t na,
-                        const uint32_t* abits, uint32_t* sub_deg, HeavyList hl, hipStream_t s);
+                        const uint32_t* abits, const uint32_t* acoarse, uint32_t* sub_deg, HeavyList hl,
+                        hipStream_t s);
+// coarse: one bit per bitmap word ((n + 31) / 32 words -> ((n + 31) / 32 + 31) / 32 entries), set when
+// the word has an alive vertex; the cut's lookups read bits and pref only under a set bit.
+void alive_coarse(const uint32_t* bits, uint32_t n, uint32_t* coarse, hipStream_t s);
 // Alive vertices of a level as a bitmap (bits: (n + 31) / 32 words) with the
 // popcount of every word in cnt (words + 1 entries, the last set to 0); an
 // exclusive scan of cnt gives pref, and ranked_list writes the alive vertices
@@ -88,8 +102,8 @@ void alive_ranked_list(const uint32_t* bits, const uint32_t* pref, uint32_t n, u
 // the number of rows of the cut longer than the heavy threshold (slots / 8192
 // + 1 suffices)
 void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* abits,
-              const uint32_t* apref, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl,
-              uint64_t* cursor, uint32_t cursor_cap, hipStream_t s);
+              const uint32_t* apref, const uint32_t* acoarse, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj,
+              HeavyList hl, uint64_t* cursor, uint32_t cursor_cap, hipStream_t s);
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
                   const uint32_t* inv, uint32_t nc, uint32_t* dout, hipStream_t s);
 void orient_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
@@ -143,7 +157,8 @@ void round_start(Ctl* ctl, uint32_t r, uint64_t alive, int32_t thr, const RoundR
                  const SearchRule& search, hipStream_t s);
 void round_stamp(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, hipStream_t s);
 // `count` rounds of (peel, advance, stamp)
-void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, int count, hipStream_t s);
+void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, int count, hipStream_t s,
+                 const LaunchProbe& probe = LaunchProbe{});
 void frontier_clear(const PeelArgs& a, const Ctl* ctl, hipStream_t s);
 void dense_eidm(const uint32_t* off_u, const uint32_t* adj_u, const uint32_t* eid_u, uint32_t nc,
                 uint32_t* eidm, hipStream_t s);

@@ -117,6 +117,32 @@ struct EntrySrc {
   }
 };
 
+// Runs of equal keys in consecutive lanes (sorted inputs put a row's entries
+// next to each other): the first lane of each run adds the whole run with
+// one LDS atomic instead of one per lane, which would serialise on one
+// address. Inactive lanes end runs. head: first lane of this lane's run;
+// len: the run's length (meaningful at the head).
+struct LaneRun {
+  int head;
+  uint32_t len;
+  bool is_head;
+};
+__device__ __forceinline__ LaneRun lane_run(uint32_t key, bool active) {
+  const int lane = lane64();
+  const uint32_t prev = __shfl_up(key, 1);
+  const bool prev_active = __shfl_up(int(active), 1) != 0;
+  const bool head = active && (lane == 0 || !prev_active || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long ends = heads | ~__ballot(active);  // a run stops at the next head or inactive lane
+  const unsigned long long le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  LaneRun r;
+  r.is_head = head;
+  r.head = 63 - __clzll((long long)(heads & le));  // valid for active lanes (lane 0 of a run is a head)
+  const unsigned long long after = ends & ~le;
+  r.len = uint32_t((after ? __ffsll((unsigned long long)after) - 1 : 64) - lane);
+  return r;
+}
+
 template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_part_hist(EntrySrc<Mode> src, uint64_t tile, uint32_t n,
                                                       uint32_t S, uint32_t B, uint32_t T, uint32_t* hist,
@@ -127,15 +153,20 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(EntrySrc<Mode> src, uint64
   __syncthreads();
   const uint64_t lo = uint64_t(t) * tile;
   const uint64_t hi = min(src.items, lo + tile);
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    uint32_t rows[2], cols[2];
-    src.get(i, rows, cols);
+  for (uint64_t i0 = lo; i0 < hi; i0 += blockDim.x) {  // block-uniform trip count: every lane reaches the shuffles
+    const uint64_t i = i0 + threadIdx.x;
+    uint32_t rows[2] = {0, 0}, cols[2] = {0, 0};
+    const bool in = i < hi;
+    if (in) src.get(i, rows, cols);
     for (int k = 0; k < src.per_item(); ++k) {
-      if (rows[k] >= n || cols[k] >= n) {
+      bool ok = in;
+      if (in && (rows[k] >= n || cols[k] >= n)) {
         atomicOr(err, 1u);
-        continue;
+        ok = false;
       }
-      atomicAdd(&h[rows[k] >> S], 1u);
+      const uint32_t b = rows[k] >> S;
+      const LaneRun r = lane_run(b, ok);
+      if (r.is_head) atomicAdd(&h[b], r.len);
     }
   }
   __syncthreads();
@@ -152,13 +183,20 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(EntrySrc<Mode> src, uin
   __syncthreads();
   const uint64_t lo = uint64_t(t) * tile;
   const uint64_t hi = min(src.items, lo + tile);
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    uint32_t rows[2], cols[2];
-    src.get(i, rows, cols);
+  const int lane = lane64();
+  for (uint64_t i0 = lo; i0 < hi; i0 += blockDim.x) {  // block-uniform trip count: every lane reaches the shuffles
+    const uint64_t i = i0 + threadIdx.x;
+    uint32_t rows[2] = {0, 0}, cols[2] = {0, 0};
+    const bool in = i < hi;
+    if (in) src.get(i, rows, cols);
     for (int k = 0; k < src.per_item(); ++k) {
-      if (rows[k] >= n || cols[k] >= n) continue;
-      const unsigned long long p = atomicAdd(&cur[rows[k] >> S], 1ull);
-      part[p] = make_uint2(rows[k], cols[k]);
+      const bool ok = in && rows[k] < n && cols[k] < n;
+      const uint32_t b = rows[k] >> S;
+      const LaneRun r = lane_run(b, ok);
+      unsigned long long base = 0;
+      if (r.is_head) base = atomicAdd(&cur[b], (unsigned long long)r.len);
+      base = __shfl(base, ok ? r.head : lane);
+      if (ok) part[base + uint64_t(lane - r.head)] = make_uint2(rows[k], cols[k]);
     }
   }
 }
@@ -178,7 +216,13 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket_rows(const uint2* __res
   const uint64_t beg = hoff[uint64_t(b) * T], end = hoff[uint64_t(b + 1) * T];
   for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) cnt[i] = 0;
   __syncthreads();
-  for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&cnt[part[i].x - r0], 1u);
+  for (uint64_t i0 = beg; i0 < end; i0 += blockDim.x) {  // block-uniform trip count
+    const uint64_t i = i0 + threadIdx.x;
+    const bool in = i < end;
+    const uint32_t r = in ? part[i].x - r0 : 0u;
+    const LaneRun run = lane_run(r, in);
+    if (run.is_head) atomicAdd(&cnt[r], run.len);
+  }
   __syncthreads();
   // exclusive scan of cnt[0..R): each thread owns R / blockDim consecutive rows
   const uint32_t per = R / blockDim.x;
@@ -218,10 +262,15 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket_rows(const uint2* __res
     run += c;
   }
   __syncthreads();
-  for (uint64_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
-    const uint2 e = part[i];
-    const uint32_t p = atomicAdd(&cnt[e.x - r0], 1u);
-    adj[beg + p] = e.y;
+  for (uint64_t i0 = beg; i0 < end; i0 += blockDim.x) {  // block-uniform trip count
+    const uint64_t i = i0 + threadIdx.x;
+    const bool in = i < end;
+    const uint2 e = in ? part[i] : make_uint2(r0, 0u);
+    const LaneRun run = lane_run(e.x - r0, in);
+    uint32_t p = 0;
+    if (run.is_head) p = atomicAdd(&cnt[e.x - r0], run.len);
+    p = __shfl(p, in ? run.head : lane);
+    if (in) adj[beg + p + uint32_t(lane - run.head)] = e.y;
   }
 }
 
@@ -409,6 +458,199 @@ __global__ __launch_bounds__(kBlock) void k_core_push(const uint64_t* __restrict
   }
 }
 
+// ------------------------------------------------- vertex-peel cascade ---
+//
+// One threshold k in one launch. The k-core does not depend on the order in
+// which vertices of degree < k are removed, so instead of synchronous rounds
+// (a sweep, a push and a close per round, hundreds of rounds on power-law
+// graphs) the removals cascade through a work queue: the sweep queues every
+// alive vertex of degree < k, and a wave that takes a vertex decrements its
+// alive neighbours with returning atomics; the one decrement that takes a
+// neighbour's degree from k to k - 1 queues that neighbour (each vertex is
+// queued exactly once: by the sweep, or at that crossing). Rows longer than
+// kCascadeChunk slots are split into chunk items when taken.
+//
+// Every neighbour is decremented, removed or not, and lvl is never read: a
+// vertex's counter then always equals its number of not-yet-processed
+// neighbours, which only falls, and a removed vertex's counter was already
+// below its removal threshold <= k, so only an alive vertex can return
+// exactly k. (A stamp read from lvl in the same launch could be one left in
+// a cache by an earlier launch, so the decision rests on the returning
+// atomic alone.)
+//
+// Queue protocol (counters in Ctl): producers reserve slots with vq_reserve
+// and then store their items; a wave claims the next slot with vq_head,
+// waits until that slot is reserved and its item stored, processes it, and
+// only then adds 1 to vq_done. A wave acts only on values it reads with
+// device-coherent atomics (the polled counter or slot itself, or the result
+// of a returning atomic), so no acquire or release fence (and no L2
+// invalidation or write-back) is needed: every reservation of an item was
+// returned before its done increment is issued, so a wave that reads
+// vq_done == vq_reserve (done first, reserve after it) sees no item in
+// flight, and none can be added any more: every wave leaves there. An item
+// past the capacity or a wait past kSpinLimit sets vq_err, which makes every
+// wave leave too (the host then fails the solve).
+constexpr uint64_t kCascadeChunk = 4096;
+constexpr unsigned long long kItemEmpty = ~0ull;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint32_t ld_coherent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_coherent(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_item(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_vcascade_start(Ctl* ctl) {
+  if (threadIdx.x != 0) return;
+  ctl->vq_reserve = 0;
+  ctl->vq_head = 0;
+  ctl->vq_done = 0;
+  ctl->vq_chunks = 0;
+  ctl->vq_err = 0;
+}
+
+// Initial frontier: every alive vertex of degree < k, stamped lvl = k.
+__global__ __launch_bounds__(kBlock) void k_core_sweep_items(uint32_t* lvl, const uint32_t* __restrict__ deg,
+                                                             uint32_t n, uint32_t k, unsigned long long* q,
+                                                             Ctl* ctl) {
+  tile_compact(
+      n, [&](uint64_t v) { return lvl[v] == kNone && deg[v] < k; },
+      [&](uint32_t at, uint64_t v) {
+        q[at] = v;  // chunk 0 of vertex v
+        lvl[v] = k;
+      },
+      &ctl->vq_reserve);
+}
+
+// Every wave-uniform value is moved to a scalar register (readfirstlane),
+// so the loops below branch on scalars and every lane stays in them; the
+// wave leaves through one exit only.
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__global__ __launch_bounds__(kBlock) void k_core_cascade(const uint64_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ adj, uint32_t* lvl,
+                                                         uint32_t* deg, uint32_t k, unsigned long long* q,
+                                                         uint32_t cap, Ctl* ctl, unsigned long long* stats) {
+  const int lane = lane64();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t quit = 0;
+  while (!quit) {
+    uint32_t h = 0;
+    if (lane == 0) h = atomicAdd(&ctl->vq_head, 1u);
+    h = uni(__shfl(h, 0));
+    // wait until slot h is reserved (go), or nothing is in flight or an error
+    // was raised (quit)
+    uint32_t go = 0, spins = 0;
+    while (!go && !quit) {
+      uint32_t st = 0;
+      if (lane == 0) {
+        const uint32_t err = ld_coherent(&ctl->vq_err);
+        const uint32_t d = ld_coherent(&ctl->vq_done);
+        // reserve must be read after done has arrived: the empty asm takes d
+        // as an input (the compiler waits for its load) and yields the zero
+        // offset of the reserve load, which therefore issues after it
+        uint32_t z = 0;
+        asm volatile("" : "+v"(z) : "v"(d));
+        const uint32_t r = ld_coherent(&ctl->vq_reserve + z);
+        st = err ? 2u : (h < r ? 1u : (d == r ? 2u : 0u));
+      }
+      st = uni(__shfl(st, 0));
+      go = st == 1u;
+      quit = st == 2u;
+      if (st == 0u) {
+        if (++spins > kSpinLimit) {
+          if (lane == 0) atomicOr(&ctl->vq_err, kVqStall);
+          quit = 1;
+        } else {
+          __builtin_amdgcn_s_sleep(8);
+        }
+      }
+    }
+    if (quit) break;
+    if (h >= cap) {  // producers never store past cap; a reservation past it is an error
+      if (lane == 0) atomicOr(&ctl->vq_err, kVqOverflow);
+      break;
+    }
+    // the slot's producer stores its item right after reserving it
+    uint32_t ilo = 0, ihi = 0;
+    spins = 0;
+    for (;;) {
+      uint32_t lo = 0, hi = 0, err = 0;
+      if (lane == 0) {
+        const unsigned long long x = ld_coherent(&q[h]);
+        lo = uint32_t(x);
+        hi = uint32_t(x >> 32);
+        if (x == kItemEmpty) err = ld_coherent(&ctl->vq_err);
+      }
+      ilo = uni(__shfl(lo, 0));
+      ihi = uni(__shfl(hi, 0));
+      err = uni(__shfl(err, 0));
+      if (!(ilo == 0xFFFFFFFFu && ihi == 0xFFFFFFFFu)) break;
+      if (err || ++spins > kSpinLimit) {
+        if (lane == 0) atomicOr(&ctl->vq_err, kVqStall);
+        quit = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (quit) break;
+    const uint32_t v = ilo, c = ihi;
+    const uint64_t rb = off[v], re = off[v + 1];
+    if (c == 0 && re - rb > kCascadeChunk) {  // chunks 1 .. extra of a long row
+      const uint32_t extra = uint32_t((re - rb - 1) / kCascadeChunk);
+      uint32_t base = 0;
+      if (lane == 0) {
+        base = atomicAdd(&ctl->vq_reserve, extra);
+        atomicAdd(&ctl->vq_chunks, extra);
+      }
+      base = uni(__shfl(base, 0));
+      if (uint64_t(base) + extra > cap) {
+        if (lane == 0) atomicOr(&ctl->vq_err, kVqOverflow);
+        break;
+      }
+      for (uint32_t x = uint32_t(lane); x < extra; x += kWave)
+        st_item(&q[base + x], (static_cast<unsigned long long>(x + 1) << 32) | v);
+    }
+    const uint64_t b = rb + uint64_t(c) * kCascadeChunk;
+    const uint64_t e = min(re, b + kCascadeChunk);
+    for (uint64_t j0 = b; j0 < e; j0 += kWave) {
+      const uint64_t j = j0 + uint64_t(lane);
+      uint32_t w = 0;
+      bool cross = false;
+      if (j < e) {
+        w = adj[j];
+        cross = atomicSub(&deg[w], 1u) == k;
+      }
+      const unsigned long long m = __ballot(cross);
+      if (m) {
+        const uint32_t leader = uni(uint32_t(__ffsll(m) - 1));
+        const uint32_t cnt = uni(uint32_t(__popcll(m)));
+        uint32_t base = 0;
+        if (uint32_t(lane) == leader) base = atomicAdd(&ctl->vq_reserve, cnt);
+        base = uni(__shfl(base, int(leader)));
+        if (uint64_t(base) + cnt > cap) {
+          if (lane == 0) atomicOr(&ctl->vq_err, kVqOverflow);
+          quit = 1;
+        } else if (cross) {
+          lvl[w] = k;
+          st_item(&q[base + uint32_t(__popcll(m & below))], static_cast<unsigned long long>(w));
+        }
+      }
+      if (quit) break;
+    }
+    if (quit) break;
+    if (stats && lane == 0) {  // measurement: row slots walked, vertices taken
+      atomicAdd(&stats[0], (unsigned long long)(e - b));
+      if (c == 0) atomicAdd(&stats[1], 1ull);
+    }
+    if (lane == 0) atomicAdd(&ctl->vq_done, 1u);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_core_alive_deg_heavy(const uint64_t* __restrict__ off,
                                                                  const uint32_t* __restrict__ adj,
                                                                  const uint32_t* __restrict__ lvl,
@@ -549,8 +791,17 @@ __global__ __launch_bounds__(kBlock) void k_alive_list(const uint32_t* __restric
 // pref[word] + the alive bits below it. The two arrays take n / 4 bytes, so
 // the rank lookups of a cut stay in L2 and the MALL where an n-entry id map
 // would be read from HBM.
+// A cut of a large level keeps few vertices, so most lookups find an empty
+// word: coarse holds one bit per bitmap word (word w / 32 non-empty), n / 256
+// bytes that stay in L2, and only the words it flags are read from bits and
+// pref.
+__device__ __forceinline__ bool coarse_hit(const uint32_t* __restrict__ coarse, uint32_t w) {
+  return (coarse[w >> 10] >> ((w >> 5) & 31)) & 1u;
+}
+
 __device__ __forceinline__ uint32_t alive_rank(const uint32_t* __restrict__ bits, const uint32_t* __restrict__ pref,
-                                               uint32_t w) {
+                                               const uint32_t* __restrict__ coarse, uint32_t w) {
+  if (!coarse_hit(coarse, w)) return kNone;
   const uint32_t b = bits[w >> 5], m = 1u << (w & 31);
   return (b & m) ? pref[w >> 5] + uint32_t(__popc(b & (m - 1u))) : kNone;
 }
@@ -573,6 +824,17 @@ __global__ __launch_bounds__(kBlock) void k_alive_words(const uint32_t* __restri
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) cnt[words] = 0;
+}
+
+// coarse[c] bit k = bits[32 c + k] != 0, for c < (words + 31) / 32.
+__global__ void k_alive_coarse(const uint32_t* __restrict__ bits, uint32_t words, uint32_t* coarse) {
+  const uint32_t cwords = (words + 31) / 32;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cwords; c += gridDim.x * blockDim.x) {
+    uint32_t m = 0;
+    const uint32_t k1 = min(32u, words - c * 32);
+    for (uint32_t k = 0; k < k1; ++k) m |= uint32_t(bits[c * 32 + k] != 0) << k;
+    coarse[c] = m;
+  }
 }
 
 // list[rank] = vertex, ranks in increasing vertex id.
@@ -598,8 +860,9 @@ __global__ void k_sub_degrees(const uint32_t* __restrict__ list, const uint32_t*
   }
 }
 
-__device__ __forceinline__ uint32_t alive_bit(const uint32_t* __restrict__ bits, uint32_t w) {
-  return (bits[w >> 5] >> (w & 31)) & 1u;
+__device__ __forceinline__ uint32_t alive_bit(const uint32_t* __restrict__ bits, const uint32_t* __restrict__ coarse,
+                                              uint32_t w) {
+  return coarse_hit(coarse, w) ? (bits[w >> 5] >> (w & 31)) & 1u : 0u;
 }
 
 // Degrees of the cut counted from the rows themselves: sub_deg[i] = the
@@ -610,7 +873,8 @@ __device__ __forceinline__ uint32_t alive_bit(const uint32_t* __restrict__ bits,
 __global__ __launch_bounds__(kBlock) void k_list_alive_deg(const uint64_t* __restrict__ off,
                                                            const uint32_t* __restrict__ adj,
                                                            const uint32_t* __restrict__ list, uint32_t na,
-                                                           const uint32_t* __restrict__ abits, uint32_t* sub_deg,
+                                                           const uint32_t* __restrict__ abits,
+                                                           const uint32_t* __restrict__ acoarse, uint32_t* sub_deg,
                                                            uint32_t* heavy, uint32_t* heavy_count) {
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   const int lane = lane64();
@@ -625,7 +889,7 @@ __global__ __launch_bounds__(kBlock) void k_list_alive_deg(const uint64_t* __res
       continue;
     }
     uint32_t c = 0;
-    for (uint64_t j = b + lane; j < e; j += kWave) c += alive_bit(abits, adj[j]);
+    for (uint64_t j = b + lane; j < e; j += kWave) c += alive_bit(abits, acoarse, adj[j]);
     c = wave_sum(c);
     if (lane == 0) sub_deg[i] = c;
   }
@@ -635,6 +899,7 @@ __global__ __launch_bounds__(kBlock) void k_list_alive_deg_heavy(const uint64_t*
                                                                  const uint32_t* __restrict__ adj,
                                                                  const uint32_t* __restrict__ list,
                                                                  const uint32_t* __restrict__ abits,
+                                                                 const uint32_t* __restrict__ acoarse,
                                                                  const uint32_t* heavy,
                                                                  const uint32_t* heavy_count,
                                                                  const uint32_t* __restrict__ slices,
@@ -649,7 +914,7 @@ __global__ __launch_bounds__(kBlock) void k_list_alive_deg_heavy(const uint64_t*
     const uint64_t b = off[v] + uint64_t(t - slices[h]) * kSlice;
     const uint64_t e = min(off[v + 1], b + kSlice);
     uint32_t c = 0;
-    for (uint64_t j = b + threadIdx.x; j < e; j += blockDim.x) c += alive_bit(abits, adj[j]);
+    for (uint64_t j = b + threadIdx.x; j < e; j += blockDim.x) c += alive_bit(abits, acoarse, adj[j]);
     c = wave_sum(c);
     if (lane64() == 0 && c) atomicAdd(&sub_deg[i], c);
   }
@@ -663,6 +928,7 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __res
                                                            const uint32_t* __restrict__ list,
                                                            const uint32_t* __restrict__ abits,
                                                            const uint32_t* __restrict__ apref,
+                                                           const uint32_t* __restrict__ acoarse,
                                                            const uint64_t* __restrict__ sub_off,
                                                            uint32_t* sub_adj, const uint32_t* heavy,
                                                            const uint32_t* heavy_count,
@@ -680,7 +946,7 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __res
     const uint64_t e = min(off[v + 1], b + kSlice);
     for (uint64_t j0 = b + (threadIdx.x & ~63u); j0 < e; j0 += blockDim.x) {
       const uint64_t j = j0 + lane;
-      const uint32_t nid = j < e ? alive_rank(abits, apref, adj[j]) : kNone;
+      const uint32_t nid = j < e ? alive_rank(abits, apref, acoarse, adj[j]) : kNone;
       const bool p = nid != kNone;
       const uint64_t mask = __ballot(p);
       if (!mask) continue;
@@ -697,7 +963,8 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict_
                                                      const uint32_t* __restrict__ adj,
                                                      const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ abits,
-                                                     const uint32_t* __restrict__ apref, uint32_t na,
+                                                     const uint32_t* __restrict__ apref,
+                                                     const uint32_t* __restrict__ acoarse, uint32_t na,
                                                      const uint64_t* __restrict__ sub_off,
                                                      uint32_t* sub_adj, uint32_t* heavy,
                                                      uint32_t* heavy_count) {
@@ -713,7 +980,7 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict_
     }
     for (uint64_t j0 = off[v]; j0 < e; j0 += kWave) {
       const uint64_t j = j0 + lane;
-      const uint32_t nid = j < e ? alive_rank(abits, apref, adj[j]) : kNone;
+      const uint32_t nid = j < e ? alive_rank(abits, apref, acoarse, adj[j]) : kNone;
       const bool p = nid != kNone;
       const uint64_t mask = __ballot(p);
       if (p) sub_adj[pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = nid;
@@ -1355,6 +1622,33 @@ void core_round(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t* 
   }
   k_vround_close<<<1, kWave, 0, s>>>(ctl);
 }
+uint64_t cascade_capacity(uint32_t n, uint64_t slots) { return uint64_t(n) + slots / kCascadeChunk + 2; }
+void core_cascade(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t k, uint32_t* lvl, uint32_t* deg,
+                  unsigned long long* q, uint32_t cap, Ctl* ctl, unsigned long long* stats, hipEvent_t ev0,
+                  hipEvent_t ev1, hipStream_t s) {
+  fill_u64(reinterpret_cast<uint64_t*>(q), cap, kItemEmpty, s);
+  k_vcascade_start<<<1, kWave, 0, s>>>(ctl);
+  if (!n) return;
+  k_core_sweep_items<<<grid_tiles(n), kBlock, 0, s>>>(lvl, deg, n, k, q, ctl);
+  // enough waves to keep a wide cascade busy; idle ones leave at quiescence
+  const int grid = int(std::min<uint64_t>(1024, std::max<uint64_t>(8, (uint64_t(n) + 255) / 256)));
+  if (ev0 && ev1) {  // measurement: events stamped at the kernel's own start and end
+    const uint64_t* a_off = off;
+    const uint32_t* a_adj = adj;
+    uint32_t* a_lvl = lvl;
+    uint32_t* a_deg = deg;
+    uint32_t a_k = k;
+    unsigned long long* a_q = q;
+    uint32_t a_cap = cap;
+    Ctl* a_ctl = ctl;
+    unsigned long long* a_stats = stats;
+    void* args[] = {&a_off, &a_adj, &a_lvl, &a_deg, &a_k, &a_q, &a_cap, &a_ctl, &a_stats};
+    KT_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(&k_core_cascade), dim3(grid), dim3(kBlock), args, 0, s,
+                              ev0, ev1, 0));
+  } else {
+    k_core_cascade<<<grid, kBlock, 0, s>>>(off, adj, lvl, deg, k, q, cap, ctl, stats);
+  }
+}
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, uint32_t n,
                         uint32_t* deg, HeavyList hl, hipStream_t s) {
   if (!n) return;
@@ -1394,14 +1688,20 @@ void sub_degrees(const uint32_t* list, const uint32_t* deg, const uint32_t* to_o
   if (na) k_sub_degrees<<<grid_for(na), kBlock, 0, s>>>(list, deg, to_orig, na, sub_deg, sub_orig);
 }
 void list_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* list, uint32_t na,
-                        const uint32_t* abits, uint32_t* sub_deg, HeavyList hl, hipStream_t s) {
+                        const uint32_t* abits, const uint32_t* acoarse, uint32_t* sub_deg, HeavyList hl,
+                        hipStream_t s) {
   if (!na) return;
   fill_u32(hl.count, 1, 0, s);
   k_list_alive_deg<<<std::min(grid_for(uint64_t(na) * kWave), 8192), kBlock, 0, s>>>(off, adj, list, na, abits,
-                                                                                     sub_deg, hl.items, hl.count);
+                                                                                     acoarse, sub_deg, hl.items,
+                                                                                     hl.count);
   k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, list, hl.slices, nullptr);
-  k_list_alive_deg_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, list, abits, hl.items, hl.count, hl.slices,
-                                                       sub_deg);
+  k_list_alive_deg_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, list, abits, acoarse, hl.items, hl.count,
+                                                       hl.slices, sub_deg);
+}
+void alive_coarse(const uint32_t* bits, uint32_t n, uint32_t* coarse, hipStream_t s) {
+  const uint32_t words = (n + 31) / 32;
+  if (words) k_alive_coarse<<<grid_for((words + 31) / 32), kBlock, 0, s>>>(bits, words, coarse);
 }
 void alive_words(const uint32_t* lvl, uint32_t n, uint32_t* bits, uint32_t* cnt, hipStream_t s) {
   const uint32_t words = (n + 31) / 32;
@@ -1412,15 +1712,15 @@ void alive_ranked_list(const uint32_t* bits, const uint32_t* pref, uint32_t n, u
   if (words) k_alive_ranked_list<<<grid_for(words), kBlock, 0, s>>>(bits, pref, words, list);
 }
 void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* abits,
-              const uint32_t* apref, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj, HeavyList hl,
-              uint64_t* cursor, uint32_t cursor_cap, hipStream_t s) {
+              const uint32_t* apref, const uint32_t* acoarse, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj,
+              HeavyList hl, uint64_t* cursor, uint32_t cursor_cap, hipStream_t s) {
   if (!na) return;
   fill_u32(hl.count, 1, 0, s);
   fill_u64(cursor, cursor_cap, 0, s);
-  k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, list, abits, apref, na, sub_off, sub_adj,
-                                                               hl.items, hl.count);
+  k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, list, abits, apref, acoarse, na, sub_off,
+                                                               sub_adj, hl.items, hl.count);
   k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, list, hl.slices, nullptr);
-  k_sub_fill_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, list, abits, apref, sub_off, sub_adj, hl.items,
+  k_sub_fill_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, list, abits, apref, acoarse, sub_off, sub_adj, hl.items,
                                                  hl.count, hl.slices, reinterpret_cast<unsigned long long*>(cursor));
 }
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to, const uint32_t* inv,

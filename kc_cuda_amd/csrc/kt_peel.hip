@@ -232,8 +232,11 @@ __device__ __forceinline__ bool dec_crosses(uint32_t x, int32_t thr, int32_t* su
   return x != kNone && atomicSub(&sup[x], 1) == thr;
 }
 
-// Walked round on a bit-matrix core. One wave per (frontier edge, 64-word
-// chunk of its rows): the common neighbours w of e = (u, v) in the chunk are
+// Walked round on a bit-matrix core. One wave per (frontier edge, chunk of
+// its rows); a chunk is 64 words, or 32 or 16 when the frontier is too small
+// to give every wave of the grid an item (a short cascade round then spreads
+// each edge's triangles over more waves). The common neighbours w of
+// e = (u, v) in the chunk are
 // the set bits of live[u] & live[v]; their count is scanned across the wave
 // and the triangles are dealt out evenly, one per lane per step, whatever
 // words they sit in. Frontier bits say at word level whether (u, w) and
@@ -250,7 +253,8 @@ __global__ __launch_bounds__(kBlock) void k_dense_peel_round(
     const unsigned long long* __restrict__ bits, const unsigned long long* __restrict__ fbits, uint32_t ws,
     const uint32_t* __restrict__ eidm, uint32_t nc, const unsigned long long* __restrict__ bits0,
     const uint32_t* __restrict__ pref0, const uint32_t* __restrict__ off_u, const uint32_t* __restrict__ eid_u,
-    const uint32_t* __restrict__ state, int32_t* sup, uint32_t mc) {
+    const uint32_t* __restrict__ state, int32_t* sup, uint32_t mc, const uint32_t* __restrict__ off_d,
+    unsigned long long* stats) {
   const uint32_t action = ctl->action;
   if (action == kRoundSweep) {
     sweep_frontier(q0, q1, ctl, state, sup, mc);
@@ -267,9 +271,11 @@ __global__ __launch_bounds__(kBlock) void k_dense_peel_round(
   __shared__ uint32_t s_inc[kBlock];
   const int lane = lane64();
   const uint32_t wbase = threadIdx.x & ~uint32_t(kWave - 1);
-  const uint32_t nch = (ws + kWave - 1) / kWave;
-  const uint64_t items = uint64_t(cnt) * nch;
   const uint64_t nw = uint64_t(gridDim.x) * (kBlock / kWave);
+  uint32_t cw = kWave;  // words per chunk
+  while (cw > 16 && uint64_t(cnt) * ((ws + cw - 1) / cw) < nw) cw >>= 1;
+  const uint32_t nch = (ws + cw - 1) / cw;
+  const uint64_t items = uint64_t(cnt) * nch;
   auto eid_of = [&](uint32_t a, uint32_t w) -> uint32_t {
     if (kEidm) return eidm[uint64_t(a) * nc + w];
     const uint64_t ra = uint64_t(a) * ws;
@@ -281,10 +287,14 @@ __global__ __launch_bounds__(kBlock) void k_dense_peel_round(
     const uint32_t qi = uint32_t(it / nch), ch = uint32_t(it - uint64_t(qi) * nch);
     const uint32_t e = q[qi];
     const uint32_t u = src[e], v = dst[e];
+    if (stats && ch == 0 && lane == 0) {  // measurement: the edge's merge work, d+(u) + d+(v)
+      atomicAdd(&stats[0], (unsigned long long)(off_d[u + 1] - off_d[u] + off_d[v + 1] - off_d[v]));
+      atomicAdd(&stats[1], 1ull);
+    }
     const uint64_t ru = uint64_t(u) * ws, rv = uint64_t(v) * ws;
-    const uint32_t i = ch * kWave + uint32_t(lane);
+    const uint32_t i = ch * cw + uint32_t(lane);
     unsigned long long t = 0, fa = 0, fb = 0;
-    if (i < ws) {
+    if (uint32_t(lane) < cw && i < ws) {
       t = bits[ru + i] & bits[rv + i];
       fa = fbits[ru + i] & t;
       fb = fbits[rv + i] & t;
@@ -316,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_peel_round(
         }
         const unsigned long long tw = s_t[wbase + lo];
         const uint32_t b = select_bit(tw, j - (s_inc[wbase + lo] - uint32_t(__popcll(tw))));
-        const uint32_t w = ((ch * kWave + lo) << 6) + b;
+        const uint32_t w = ((ch * cw + lo) << 6) + b;
         const bool ina = (s_fa[wbase + lo] >> b) & 1ull, inb = (s_fb[wbase + lo] >> b) & 1ull;
         if (!ina && !inb) {
           xa = eid_of(u, w);
@@ -484,7 +494,8 @@ static void launch_stamp(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, hip
 
 void round_stamp(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, hipStream_t s) { launch_stamp(a, ctl, rule, s); }
 
-void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, int count, hipStream_t s) {
+void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, int count, hipStream_t s,
+                 const LaunchProbe& probe) {
   auto bits = reinterpret_cast<unsigned long long*>(a.bits);
   auto fbits = reinterpret_cast<unsigned long long*>(a.fbits);
   auto b0 = reinterpret_cast<const unsigned long long*>(a.bits0);
@@ -492,14 +503,25 @@ void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, 
   const int advance_grid = rule.dense ? work_grid(a.mc, 4 * kBlock, kAdvanceGrid) : 1;
   for (int i = 0; i < count; ++i) {
     if (rule.dense) {
-      if (a.eidm)
-        k_dense_peel_round<true><<<peel_grid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst, bits, fbits, a.ws, a.eidm,
-                                                              a.nc, b0, a.pref0, a.off_u, a.eid_u, a.state, a.sup,
-                                                              a.mc);
+      // every argument as an lvalue for the event-stamped launch
+      uint32_t *q0 = a.q0, *q1 = a.q1;
+      Ctl* c = ctl;
+      const uint32_t *src = a.src, *dst = a.dst, *eidm = a.eidm, *pref0 = a.pref0, *off_u = a.off_u,
+                     *eid_u = a.eid_u, *state = a.state, *off_d = a.off_d;
+      unsigned long long *lb = bits, *fb = fbits;
+      const unsigned long long* b0c = b0;
+      uint32_t ws = a.ws, nc = a.nc, mc = a.mc;
+      int32_t* sup = a.sup;
+      unsigned long long* st = probe.stats;
+      void* args[] = {&q0, &q1, &c, &src, &dst, &lb, &fb, &ws, &eidm, &nc, &b0c, &pref0, &off_u, &eid_u,
+                      &state, &sup, &mc, &off_d, &st};
+      const void* fn = a.eidm ? reinterpret_cast<const void*>(&k_dense_peel_round<true>)
+                              : reinterpret_cast<const void*>(&k_dense_peel_round<false>);
+      if (probe.ev)
+        KT_HIP(hipExtLaunchKernel(fn, dim3(peel_grid), dim3(kBlock), args, 0, s, probe.ev[2 * i],
+                                  probe.ev[2 * i + 1], 0));
       else
-        k_dense_peel_round<false><<<peel_grid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst, bits, fbits, a.ws,
-                                                               a.eidm, a.nc, b0, a.pref0, a.off_u, a.eid_u, a.state,
-                                                               a.sup, a.mc);
+        KT_HIP(hipLaunchKernel(fn, dim3(peel_grid), dim3(kBlock), args, 0, s));
     } else {
 #define KT_SPARSE_ROUND(G)                                                                                 \
   k_sparse_peel_round<G><<<peel_grid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.off_u, a.adj_u, a.eid_u, a.src, a.dst, \

@@ -156,7 +156,14 @@ class Solver {
   // each launch's start and end, bit 2 counts the light row slots it walks
   // and the vertices it takes (separate solves keep the counters' atomics
   // out of the timing); push_stats sums them since set_push_probe.
-  void set_push_probe(int mode);
+  void set_push_probe(int mode) { set_probe(kProbeVertexPeel, mode); }
+  // kernel: kProbeVertexPeel (the vertex-peel cascade, or the push of its
+  // rounds) or kProbeDensePeel (the bit-matrix edge-peel round); for the
+  // latter the counters are the frontier edges' d+(u) + d+(v) and their
+  // number.
+  static constexpr int kProbeVertexPeel = 0;
+  static constexpr int kProbeDensePeel = 1;
+  void set_probe(int kernel, int mode);
   void push_stats(uint64_t* launches, double* ms, uint64_t* slots, uint64_t* vertices);
   // Settings the batch autotuner varies (every value computes the same result).
   struct Tuning {
@@ -187,6 +194,12 @@ class Solver {
   // stages (kt_stages.hip)
   void load_csr_host(uint32_t n, const uint64_t* off, const uint32_t* adj);
   uint64_t peel_vertices(LevelGraph& g, uint32_t k);
+  uint64_t peel_vertices_rounds(LevelGraph& g, uint32_t k);
+  // vertex peel: 0 by level size (one cascade launch per threshold up to
+  // kCascadeMaxSlots row slots, synchronous rounds above), 1 always rounds,
+  // 2 always the cascade
+  void set_vertex_mode(int m) { vertex_mode_ = m; }
+  static constexpr uint64_t kCascadeMaxSlots = uint64_t(1) << 24;
   // fresh: start from full degrees with every vertex alive; otherwise from
   // the state already on level 0. allow_cut: continue on induced subgraphs.
   uint32_t max_core(bool allow_cut = true, bool fresh = true);
@@ -280,7 +293,9 @@ class Solver {
   bool force_edge_count_ = false;
   bool force_atomic_build_ = false;
   int core_mode_ = 0;
-  DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_, dhist_;
+  DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_, dhist_, vq_;
+  int vertex_mode_ = 0;        // see set_vertex_mode
+  bool vertex_rounds_ = false;  // the last vertex peel ran synchronous rounds (reported by last_stats)
   std::vector<uint64_t> deg_at_least_;  // input degree histogram, suffix sums per bin (empty: none)
   DevBuf bk_state_, bk_sup_, bk_bits_, bk_del_;  // threshold-probe backups
   uint64_t eidm_budget_ = uint64_t(4) << 30;  // bytes for the edge-id matrix
@@ -292,7 +307,9 @@ class Solver {
   bool gallop_ = true;
   uint32_t count_kind_ = 0, count_words_ = 0;  // the last timed support count
   uint32_t walk_levels_ = 16;
-  int probe_push_ = 0;               // measurement of the push kernel (mode bits)
+  int probe_push_ = 0;               // measurement of a kernel (mode bits)
+  int probe_kernel_ = kProbeVertexPeel;
+  hipEvent_t* probe_events(int launches);  // 2 * launches events from the pool
   std::vector<hipEvent_t> probe_ev_;
   size_t probe_used_ = 0;
   DevBuf probe_stats_;

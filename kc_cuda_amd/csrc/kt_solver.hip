@@ -119,6 +119,8 @@ void Solver::init_common() {
   if (const char* v = std::getenv("KT_TRACE")) trace_ = v[0] == '1';
   if (const char* v = std::getenv("KT_EIDM_BUDGET_MB")) eidm_budget_ = uint64_t(std::atoll(v)) << 20;
   if (const char* v = std::getenv("KT_WALK_LEVELS")) walk_levels_ = std::max(1, std::atoi(v));
+  if (const char* v = std::getenv("KT_VERTEX_PEEL"))
+    vertex_mode_ = std::strcmp(v, "rounds") == 0 ? 1 : std::strcmp(v, "cascade") == 0 ? 2 : 0;
   default_drop_ratio_ = sparse_drop_ratio_;
 }
 
@@ -129,7 +131,7 @@ void Solver::each_buffer(F f) const {
                          &adj_d_, &src_, &din_, &din_off_, &tkeys_, &tvals_, &tiota_, &in_src_, &off_u_,
                          &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &truss_out_, &del_thr_, &orig0_,
                          &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &fbits_, &pref0_, &eidm_, &heavy_,
-                         &vflag_, &vmap_, &roff_, &dhist_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_, &text_,
+                         &vflag_, &vmap_, &roff_, &dhist_, &vq_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_, &text_,
                          &tile_cnt_, &tile_off_, &lx_, &ly_, &lkind_, &lx2_, &ly2_, &lkeep_, &lkeep_off_,
                          &lstat_, &probe_stats_};
   for (const DevBuf* b : all) f(*b);
@@ -338,9 +340,22 @@ LevelGraph& Solver::level0() {
   return g;
 }
 
-void Solver::set_push_probe(int mode) {
+hipEvent_t* Solver::probe_events(int launches) {
+  while (2 * (probe_used_ + size_t(launches)) > probe_ev_.size())
+    for (int e = 0; e < 64; ++e) {
+      probe_ev_.push_back(nullptr);
+      KT_HIP(hipEventCreate(&probe_ev_.back()));
+    }
+  hipEvent_t* ev = probe_ev_.data() + 2 * probe_used_;
+  probe_used_ += size_t(launches);
+  return ev;
+}
+
+void Solver::set_probe(int kernel, int mode) {
   KT_HIP(hipSetDevice(device_));
   KT_HIP(hipStreamSynchronize(stream_));
+  if (kernel != kProbeVertexPeel && kernel != kProbeDensePeel) fail("probe: unknown kernel");
+  probe_kernel_ = kernel;
   probe_push_ = mode & 3;
   probe_used_ = 0;
   if (probe_push_) {

@@ -39,11 +39,42 @@ constexpr uint32_t kMinCut = 256;  // smaller alive sets stay on their level
 
 // ------------------------------------------------------ (b) vertex peel ---
 
-// Removes every alive vertex of degree < k, round by round; returns the
-// number removed. lvl[] of each removed vertex becomes k.
-// Rounds run on the device in batches; the host reads the control block
-// between batches only.
+// Removes every alive vertex of degree < k; returns the number removed.
+// lvl[] of each removed vertex becomes k. A level of up to kCascadeMaxSlots
+// row slots is peeled by one cascade launch (a work queue; the k-core does
+// not depend on the removal order), which saves the per-round launches and
+// readbacks that dominate small graphs; a larger one by synchronous rounds,
+// run on the device in batches with the host reading the control block
+// between batches, whose fire-and-forget decrements stream faster than the
+// cascade's returning ones (set_vertex_mode forces either).
 uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
+  if (g.n == 0) return 0;
+  vertex_rounds_ = vertex_mode_ == 1 || (vertex_mode_ == 0 && g.slots > kCascadeMaxSlots);
+  if (vertex_rounds_) return peel_vertices_rounds(g, k);
+  const uint64_t cap = cascade_capacity(g.n, g.slots);
+  if (cap >= 0xFFFFFFFFull) fail("vertex peel: cascade queue exceeds 2^32 items");
+  vq_.ensure(sizeof(unsigned long long) * cap);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  unsigned long long* ps = nullptr;
+  if (probe_kernel_ == kProbeVertexPeel) {
+    if (probe_push_ & 1) {  // measurement: events at the launch's start and end
+      hipEvent_t* ev = probe_events(1);
+      e0 = ev[0];
+      e1 = ev[1];
+    }
+    if (probe_push_ & 2) ps = probe_stats_.as<unsigned long long>();  // measurement: slot counters
+  }
+  core_cascade(g.off, g.adj, g.n, k, g.lvl, g.deg, vq_.as<unsigned long long>(), uint32_t(cap), ctl_, ps, e0, e1,
+               stream_);
+  const Ctl h = read_ctl();
+  if (h.vq_err) fail(h.vq_err & kVqOverflow ? "vertex peel: cascade queue overflow" : "vertex peel: cascade stalled");
+  const uint64_t removed = uint64_t(h.vq_reserve) - h.vq_chunks;
+  if (trace_) std::fprintf(stderr, "[kt] vertex-peel level_n=%u k=%u removed=%llu\n", g.n, k, (unsigned long long)removed);
+  if (removed > g.n || h.vq_done != h.vq_reserve) fail("vertex peel bookkeeping mismatch");
+  return removed;
+}
+
+uint64_t Solver::peel_vertices_rounds(LevelGraph& g, uint32_t k) {
   constexpr int kMaxBatch = 32;
   if (g.n == 0) return 0;
   q0_.ensure(sizeof(uint32_t) * (size_t(g.n) + 1));
@@ -56,17 +87,14 @@ uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
     for (int i = 0; i < batch; ++i) {
       hipEvent_t e0 = nullptr, e1 = nullptr;
       unsigned long long* ps = nullptr;
-      if (probe_push_ & 1) {  // measurement: events at every push launch's start and end
-        if (2 * probe_used_ + 2 > probe_ev_.size())
-          for (int e = 0; e < 64; ++e) {
-            probe_ev_.push_back(nullptr);
-            KT_HIP(hipEventCreate(&probe_ev_.back()));
-          }
-        e0 = probe_ev_[2 * probe_used_];
-        e1 = probe_ev_[2 * probe_used_ + 1];
-        ++probe_used_;
+      if (probe_kernel_ == kProbeVertexPeel) {
+        if (probe_push_ & 1) {  // measurement: events at every push launch's start and end
+          hipEvent_t* ev = probe_events(1);
+          e0 = ev[0];
+          e1 = ev[1];
+        }
+        if (probe_push_ & 2) ps = probe_stats_.as<unsigned long long>();  // measurement: slot counters
       }
-      if (probe_push_ & 2) ps = probe_stats_.as<unsigned long long>();  // measurement: slot counters
       core_round(g.off, g.adj, g.n, q, ctl_, k, g.lvl, g.deg, hl, ps, e0, e1, stream_);
     }
     queued += batch;
@@ -89,11 +117,14 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive, b
   // alive bitmap | word popcounts | their exclusive scan, then the ranked list
   const size_t words = (size_t(p.n) + 31) / 32;
   list_.ensure(sizeof(uint32_t) * (size_t(n_) + 1));
-  new_id_.ensure(sizeof(uint32_t) * 3 * (words + 1));
+  const size_t cwords = (words + 31) / 32;
+  new_id_.ensure(sizeof(uint32_t) * (3 * (words + 1) + cwords + 1));
   uint32_t* abits = new_id_.as<uint32_t>();
   uint32_t* acnt = abits + words + 1;
   uint32_t* apref = acnt + words + 1;
+  uint32_t* acoarse = apref + words + 1;
   alive_words(p.lvl, p.n, abits, acnt, stream_);
+  alive_coarse(abits, p.n, acoarse, stream_);
   exclusive_scan_u32(acnt, apref, uint32_t(words + 1));
   alive_ranked_list(abits, apref, p.n, list_.as<uint32_t>(), stream_);
   const uint32_t na = read_u32(apref + words);
@@ -111,14 +142,14 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive, b
   s.ok.assign(1, k_enter);
   fill_u32(s.deg + na, 1, 0, stream_);
   sub_degrees(list_.as<uint32_t>(), recount ? nullptr : p.deg, p.to_orig, na, s.deg, s.to_orig, stream_);
-  if (recount) list_alive_degrees(p.off, p.adj, list_.as<uint32_t>(), na, abits, s.deg, heavy(), stream_);
+  if (recount) list_alive_degrees(p.off, p.adj, list_.as<uint32_t>(), na, abits, acoarse, s.deg, heavy(), stream_);
   exclusive_scan_u32_to_u64(s.deg, s.off, na + 1);
   s.slots = read_u64(s.off + na);
   s.b_adj.ensure(sizeof(uint32_t) * std::max<uint64_t>(s.slots, 1));
   s.adj = s.b_adj.as<uint32_t>();
   const uint32_t cap = uint32_t(std::min<uint64_t>(na, p.slots / 8192 + 1));
   cursor_.ensure(sizeof(uint64_t) * (size_t(cap) + 1));
-  sub_fill(p.off, p.adj, list_.as<uint32_t>(), abits, apref, na, s.off, s.adj, heavy(), cursor_.as<uint64_t>(), cap,
+  sub_fill(p.off, p.adj, list_.as<uint32_t>(), abits, apref, acoarse, na, s.off, s.adj, heavy(), cursor_.as<uint64_t>(), cap,
            stream_);
   fill_u32(s.lvl, na, kNone, stream_);
   ++depth_;
@@ -541,6 +572,7 @@ PeelArgs Solver::peel_args(const CoreView& cv) {
   a.off_u = cv.off_u;
   a.adj_u = cv.adj_u;
   a.eid_u = cv.eid_u;
+  a.off_d = cv.off_d;
   if (cv.dense) {
     a.bits = cv.bits;
     a.fbits = cv.fbits;
@@ -593,7 +625,13 @@ TrussOutcome Solver::run_levels(CoreView* cv, int32_t thr, uint64_t alive, uint3
   for (;;) {
     if (queued > 3 * uint64_t(cv->mc) + 4 * kMaxBatch) fail("edge peel did not converge");
     const double avg = cv->nc ? double(2ull * cv->mc) / cv->nc : 0.0;
-    peel_rounds(peel_args(*cv), ctl_, round_rule(*cv), lanes_peel_ ? lanes_peel_ : auto_lanes(avg), batch, stream_);
+    LaunchProbe probe;
+    if (probe_kernel_ == kProbeDensePeel && cv->dense) {  // measurement of the dense peel round
+      if (probe_push_ & 1) probe.ev = probe_events(batch);
+      if (probe_push_ & 2) probe.stats = probe_stats_.as<unsigned long long>();
+    }
+    peel_rounds(peel_args(*cv), ctl_, round_rule(*cv), lanes_peel_ ? lanes_peel_ : auto_lanes(avg), batch, stream_,
+                probe);
     queued += batch;
     const Ctl h = read_ctl();
     if (trace_)
@@ -880,6 +918,7 @@ void Solver::last_stats(kt_stats* out) {
   out->n_core = cv.nc0;
   out->count_kernel = count_kind_;
   out->core_words = count_words_;
+  out->vertex_rounds = vertex_rounds_ ? 1u : 0u;
   out->m_core = cv.mc0;
   out->truss_edges = last_truss_;
   if (cv.mc0 == 0) return;

@@ -99,6 +99,41 @@ def test_core_representations_agree(ctx, core, seed):
         ctx.tune()
 
 
+def _vertex_cases():
+    """Graphs for the vertex-peel variants: random with cliques, a star whose
+    hub row spans many cascade chunks, two stars sharing leaves, R-MAT with
+    hub rows above 4096 slots, and ER with a planted clique."""
+    from kc_cuda_amd import gen
+    star = np.array([[0, v] for v in range(1, 20000)] + [[1, 2], [2, 3], [1, 3]], dtype=np.uint32)
+    two = np.array([[0, v] for v in range(2, 9000)] + [[1, v] for v in range(2, 9000)] + [[0, 1]],
+                   dtype=np.uint32)
+    return [("random", random_graph(700, 0.03, 77, cliques=(9, 6)), 700),
+            ("star", canon_pairs(star), 20000),
+            ("two_stars", canon_pairs(two), 9000),
+            ("rmat17", *gen.rmat(17, 16, 0.57, 0.19, 0.19, True, 20, 13)),
+            ("er_clique", *gen.er_clique(30000, 12.0, 30, 9))]
+
+
+@pytest.mark.parametrize("vertex", ["cascade", "rounds", "auto"])
+def test_vertex_peel_variants_agree(ctx, vertex):
+    """The vertex peel as one cascade launch per threshold and as
+    synchronous rounds: the same max core, k-hat, kmax and kmax-truss as the
+    oracle."""
+    O = _oracle()
+    for name, pairs, n in _vertex_cases():
+        print(f"[gpu] vertex={vertex} {name} n={n} m={len(pairs)}", flush=True)
+        ref = O.solve_pairs(pairs, n, want_edges=True)
+        try:
+            ctx.tune(vertex=vertex)
+            r = ctx.solve_pairs(pairs, n)
+            edges = ctx.truss_edges()
+        finally:
+            ctx.tune()
+        assert (r.max_core, r.k_hat, r.kmax, r.truss_edges) == (ref.max_core, ref.k_hat, ref.kmax,
+                                                                ref.truss_edges), name
+        np.testing.assert_array_equal(edges, canon_pairs(ref.truss), err_msg=name)
+
+
 @pytest.mark.parametrize("scale", [12, 16, 18, 20])
 def test_rmat_vs_golden(ctx, scale):
     from kc_cuda_amd import gen
