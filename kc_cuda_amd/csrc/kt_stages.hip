@@ -188,13 +188,17 @@ uint64_t Solver::filtered_core(uint32_t k) {
   return kept - removed;
 }
 
-// Start of the max-core search from a high threshold: an eighth of a lower
+// Start of the max-core search from a high threshold: a sixteenth of a lower
 // bound of the degree h-index H (at least H vertices have degree >= H; the
-// max core is at most H), reached by filtered_core; when its core is empty
-// the threshold drops by 4x, and when the filter would keep more than a
-// quarter of the vertices the plain search from 1 runs instead. On success
-// *gp is the cut level with the kf-core alive, *k_out = kf and *alive_out its
-// vertex count. The degree histogram stays for later filtered cuts.
+// max core is at most H), reached by filtered_core. On power-law graphs the
+// row slots of the vertices of degree >= d shrink very slowly with d, so a
+// low threshold costs the cut little more than a high one, and it leaves a
+// cut level that pass 2's (L-1)-core usually lies in (no second cut from
+// level 0). When its core is empty the threshold drops by 4x, and when the
+// filter would keep more than a quarter of the vertices the plain search
+// from 1 runs instead. On success *gp is the cut level with the kf-core
+// alive, *k_out = kf and *alive_out its vertex count. The degree histogram
+// stays for later filtered cuts.
 bool Solver::filtered_start(LevelGraph** gp, uint32_t* k_out, uint64_t* alive_out) {
   LevelGraph& g0 = **gp;
   const uint32_t n = g0.n;
@@ -212,7 +216,7 @@ bool Solver::filtered_start(LevelGraph** gp, uint32_t* k_out, uint64_t* alive_ou
     const uint64_t lo = uint32_t(b) < kDegExact ? uint64_t(b) : uint64_t(1) << (b - kDegExact + kDegExactLog2);
     hidx = std::max(hidx, std::min(deg_at_least_[b], lo));
   }
-  for (uint32_t kf = uint32_t(hidx / 8); kf >= 3; kf /= 4) {
+  for (uint32_t kf = uint32_t(hidx / 16); kf >= 3; kf /= 4) {
     if (degree_at_least(kf) > n / 4) break;
     const uint64_t alive = filtered_core(kf);
     if (trace_)
