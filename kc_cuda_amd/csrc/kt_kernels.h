@@ -23,13 +23,17 @@ void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStr
 // Partitioned build: entries bucketed by row >> S (B buckets, T tiles of
 // `tile` items), then rows built per bucket in LDS. mode 0: a = interleaved
 // pairs (both directions); mode 1: a = xs (columns), b = ys (rows).
-// hist has B*T entries (bucket-major); hoff is its exclusive scan (B*T+1).
+// hist_t has T*B entries (tile-major, one row per tile); its transpose is
+// scanned bucket-major into hoff (B*T+1), whose tile-major transpose hoff_t
+// gives each tile its write cursors.
 bool csr_partition_plan(uint32_t n, uint32_t* S, uint32_t* B);
 void csr_part_hist(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile,
-                   uint32_t n, uint32_t S, uint32_t B, uint32_t T, uint32_t* hist, uint32_t* err,
+                   uint32_t n, uint32_t S, uint32_t B, uint32_t T, uint32_t* hist_t, uint32_t* err,
                    hipStream_t s);
+void transpose_u32(const uint32_t* in, uint32_t rows, uint32_t cols, uint32_t* out, hipStream_t s);
+void transpose_u64(const uint64_t* in, uint32_t rows, uint32_t cols, uint64_t* out, hipStream_t s);
 void csr_part_scatter(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile,
-                      uint32_t n, uint32_t S, uint32_t B, uint32_t T, const uint64_t* hoff,
+                      uint32_t n, uint32_t S, uint32_t B, uint32_t T, const uint64_t* hoff_t,
                       uint32_t* part, hipStream_t s);
 void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, uint32_t B, uint32_t T, uint32_t S,
                      uint32_t n, uint64_t* off, uint32_t* deg, uint32_t* adj, hipStream_t s);

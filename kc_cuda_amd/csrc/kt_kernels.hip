@@ -145,7 +145,7 @@ __device__ __forceinline__ LaneRun lane_run(uint32_t key, bool active) {
 
 template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_part_hist(EntrySrc<Mode> src, uint64_t tile, uint32_t n,
-                                                      uint32_t S, uint32_t B, uint32_t T, uint32_t* hist,
+                                                      uint32_t S, uint32_t B, uint32_t* hist_t,
                                                       uint32_t* err) {
   extern __shared__ uint32_t h[];
   const uint32_t t = blockIdx.x;
@@ -170,16 +170,35 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(EntrySrc<Mode> src, uint64
     }
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist[uint64_t(i) * T + t] = h[i];
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist_t[uint64_t(t) * B + i] = h[i];
+}
+
+// out[c * rows + r] = in[r * cols + c], through 64 x 64 tiles in LDS (a
+// padded row keeps the column reads off one bank).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_transpose(const T* __restrict__ in, uint32_t rows, uint32_t cols,
+                                                      T* __restrict__ out) {
+  __shared__ T tile[64][65];
+  const uint32_t c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+  const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
+  for (uint32_t y = ty; y < 64; y += kBlock / 64) {
+    const uint32_t r = r0 + y, c = c0 + tx;
+    if (r < rows && c < cols) tile[y][tx] = in[uint64_t(r) * cols + c];
+  }
+  __syncthreads();
+  for (uint32_t y = ty; y < 64; y += kBlock / 64) {
+    const uint32_t c = c0 + y, r = r0 + tx;
+    if (r < rows && c < cols) out[uint64_t(c) * rows + r] = tile[tx][y];
+  }
 }
 
 template <int Mode>
 __global__ __launch_bounds__(kBlock) void k_part_scatter(EntrySrc<Mode> src, uint64_t tile, uint32_t n,
-                                                         uint32_t S, uint32_t B, uint32_t T,
-                                                         const uint64_t* __restrict__ hoff, uint2* part) {
+                                                         uint32_t S, uint32_t B,
+                                                         const uint64_t* __restrict__ hoff_t, uint2* part) {
   extern __shared__ unsigned long long cur[];
   const uint32_t t = blockIdx.x;
-  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) cur[i] = hoff[uint64_t(i) * T + t];
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) cur[i] = hoff_t[uint64_t(t) * B + i];
   __syncthreads();
   const uint64_t lo = uint64_t(t) * tile;
   const uint64_t hi = min(src.items, lo + tile);
@@ -1565,22 +1584,28 @@ bool csr_partition_plan(uint32_t n, uint32_t* S, uint32_t* B) {
   return false;
 }
 void csr_part_hist(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile, uint32_t n,
-                   uint32_t S, uint32_t B, uint32_t T, uint32_t* hist, uint32_t* err, hipStream_t s) {
+                   uint32_t S, uint32_t B, uint32_t T, uint32_t* hist_t, uint32_t* err, hipStream_t s) {
   const size_t lds = sizeof(uint32_t) * B;
   if (mode == 0)
-    k_part_hist<0><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, T, hist, err);
+    k_part_hist<0><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hist_t, err);
   else
-    k_part_hist<1><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, T, hist, err);
+    k_part_hist<1><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, hist_t, err);
+}
+void transpose_u32(const uint32_t* in, uint32_t rows, uint32_t cols, uint32_t* out, hipStream_t s) {
+  if (rows && cols) k_transpose<uint32_t><<<dim3((cols + 63) / 64, (rows + 63) / 64), kBlock, 0, s>>>(in, rows, cols, out);
+}
+void transpose_u64(const uint64_t* in, uint32_t rows, uint32_t cols, uint64_t* out, hipStream_t s) {
+  if (rows && cols) k_transpose<uint64_t><<<dim3((cols + 63) / 64, (rows + 63) / 64), kBlock, 0, s>>>(in, rows, cols, out);
 }
 void csr_part_scatter(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile,
-                      uint32_t n, uint32_t S, uint32_t B, uint32_t T, const uint64_t* hoff, uint32_t* part,
+                      uint32_t n, uint32_t S, uint32_t B, uint32_t T, const uint64_t* hoff_t, uint32_t* part,
                       hipStream_t s) {
   const size_t lds = sizeof(unsigned long long) * B;
   uint2* p = reinterpret_cast<uint2*>(part);
   if (mode == 0)
-    k_part_scatter<0><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, T, hoff, p);
+    k_part_scatter<0><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hoff_t, p);
   else
-    k_part_scatter<1><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, T, hoff, p);
+    k_part_scatter<1><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, hoff_t, p);
 }
 void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, uint32_t B, uint32_t T, uint32_t S, uint32_t n,
                      uint64_t* off, uint32_t* deg, uint32_t* adj, hipStream_t s) {

@@ -3,19 +3,21 @@
 // A level removes, round by round, every edge whose support is below the
 // threshold (reference: src/kc.cu:886-1056, __delete_edge / ktruss), and the
 // search moves from level to level (src/kc.cu:1140-1309). Here both run on
-// the device: a round is three launches that read what to do, the frontier,
+// the device: a round is two launches that read what to do, the frontier,
 // its size, the round stamp and the threshold from the control block.
 //   peel     walk the triangles of the frontier, decrement the partners,
 //            append edges that cross below the threshold to the next queue;
 //            at a level change: collect the new level's frontier;
-//   advance  clear the frontier from the live bit matrix (dense cores) and, in
-//            the last workgroup, switch queues and decide the next round;
-//   stamp    stamp the current frontier (state, deletion threshold, frontier
-//            bits); at a level's end: reduce the live supports and, in the
-//            last workgroup, record the level and set the next threshold.
+//   next     clear the walked frontier from the live bit matrix (dense
+//            cores); every workgroup takes the same decision on the next
+//            frontier from the control block and stamps it (state, deletion
+//            threshold, frontier bits), or, when it is empty, reduces the
+//            live supports; the last workgroup switches queues, records a
+//            finished level and sets what the next round does.
 // The host enqueues rounds in batches, reads the control block once per batch
 // and steps in only to drop a frontier, compact, or collect the answer; a
-// kernel whose action does not concern it does nothing.
+// kernel whose action does not concern it does nothing. k_round_stamp stamps
+// the first frontier of a search (or ends the level, when it is empty).
 #include <algorithm>
 
 #include "kt_device.h"
@@ -24,11 +26,11 @@
 namespace kt {
 namespace {
 
-// Grids of the round kernels. Stamp and advance end with one ticket atomic
-// per workgroup, so they stay small; their loops stride over the frontier.
+// Grids of the round kernels. Stamp and next end with one ticket atomic per
+// workgroup, so they stay small; their loops stride over the frontier (or
+// the edges, for a level's minimum support).
 constexpr int kStampGrid = 256;
 constexpr int kPeelGrid = 1024;
-constexpr int kAdvanceGrid = 64;
 constexpr int kClearGrid = 512;
 
 // What the next round does with a frontier of c edges when `alive` edges are
@@ -60,6 +62,7 @@ __global__ void k_round_start(Ctl* ctl, uint32_t r, unsigned long long alive, in
   ctl->thr = thr;
   ctl->ticket = 0;
   ctl->stamped = 0;
+  ctl->minsup = kNone;
   if (reset) {
     ctl->err &= ~kErrPeelCount;
     ctl->rounds_done = 0;
@@ -76,12 +79,75 @@ __global__ void k_round_start(Ctl* ctl, uint32_t r, unsigned long long alive, in
   set_action(ctl, decide_round(c, alive, thr, rule, false));
 }
 
-// Stamp the current frontier once: state = round, deletion threshold of its
-// base edge = thr, and (walked frontiers of dense cores) its frontier bits.
-// The rounds queued after a drop or an empty level find it stamped already.
-// At a level's end the same slot reduces the supports of the live edges; the
-// last workgroup records the level (kmax = min support + 2, its edge count
-// and threshold), applies the early exit and moves to threshold min + 1.
+// A level is complete (its frontier is empty): every workgroup reduces the
+// supports of the live edges; the last one (after switch_queue(): the walked
+// round's queue switch) records the level (kmax = min support + 2, its edge
+// count and threshold), applies the early exit and the level budget, and
+// moves to threshold min + 1: a sweep next, or a compaction of a sparse core
+// that has lost half its edges. ctl->minsup is kNone on entry and on exit.
+template <class Switch>
+__device__ __forceinline__ void level_end(Ctl* ctl, const uint32_t* __restrict__ state,
+                                          const int32_t* __restrict__ sup, const RoundRule& rule,
+                                          Switch switch_queue) {
+  __shared__ uint32_t s_min[kBlock / kWave];
+  uint32_t best = kNone;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < rule.mc; e += gridDim.x * blockDim.x)
+    if (state[e] == kNone) best = min(best, uint32_t(sup[e]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) best = min(best, uint32_t(__shfl_xor(best, o)));
+  if (lane64() == 0) s_min[threadIdx.x / kWave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / kWave; ++k) best = min(best, s_min[k]);
+    if (best != kNone) atomicMin(&ctl->minsup, best);
+  }
+  if (last_block(&ctl->ticket) && threadIdx.x == 0) {
+    switch_queue();
+    const uint32_t ms = ctl->minsup;
+    ctl->minsup = kNone;
+    const unsigned long long alive = ctl->alive;
+    if (ms == kNone || alive == 0) {  // nothing live: the search is over
+      if ((ms == kNone) != (alive == 0)) ctl->err |= kErrPeelCount;
+      ctl->action = kRoundDone;
+      return;
+    }
+    ctl->level_kmax = ms + 2;
+    ctl->level_count = alive;
+    ctl->level_floor = ctl->thr;
+    ctl->levels += 1;
+    if (ctl->first && ctl->early_ans && alive == ctl->early_ans && (ctl->exact_early || ms + 2 == ctl->early_k)) {
+      ctl->level_kmax = ctl->early_k;
+      ctl->action = kRoundDone;
+      return;
+    }
+    ctl->first = 0;
+    if (ctl->max_levels && ctl->levels >= ctl->max_levels) {  // level budget spent
+      ctl->action = kRoundDone;
+      return;
+    }
+    ctl->thr = int32_t(ms) + 1;
+    ctl->action = (!rule.dense && 2ull * alive <= rule.mc) ? kRoundCompact : kRoundSweep;
+  }
+}
+
+// Stamp of a frontier edge: state = round, deletion threshold of its base
+// edge = thr, and (frontiers walked on dense cores) its frontier bits.
+__device__ __forceinline__ void stamp_edge(uint32_t e, uint32_t r, uint32_t thr, uint32_t* state,
+                                           const uint32_t* __restrict__ orig, uint32_t* del_thr,
+                                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                           unsigned long long* fbits, uint32_t ws) {
+  state[e] = r;
+  del_thr[orig[e]] = thr;
+  if (fbits) {
+    const uint32_t u = src[e], v = dst[e];
+    atomicOr(&fbits[uint64_t(u) * ws + (v >> 6)], 1ull << (v & 63));
+    atomicOr(&fbits[uint64_t(v) * ws + (u >> 6)], 1ull << (u & 63));
+  }
+}
+
+// Stamp the first frontier of a search once (the rounds queued after a drop
+// or an empty level find it stamped already); with an empty frontier, the
+// level end instead.
 __global__ __launch_bounds__(kBlock) void k_round_stamp(uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t* state,
                                                         const int32_t* __restrict__ sup,
                                                         const uint32_t* __restrict__ orig, uint32_t* del_thr,
@@ -90,43 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_round_stamp(uint32_t* q0, uint32_t* 
                                                         unsigned long long* fbits, uint32_t ws, RoundRule rule) {
   const uint32_t a = ctl->action;
   if (a == kRoundMinSup) {
-    __shared__ uint32_t s_min[kBlock / kWave];
-    uint32_t best = kNone;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < rule.mc; e += gridDim.x * blockDim.x)
-      if (state[e] == kNone) best = min(best, uint32_t(sup[e]));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) best = min(best, uint32_t(__shfl_xor(best, o)));
-    if (lane64() == 0) s_min[threadIdx.x / kWave] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int k = 1; k < kBlock / kWave; ++k) best = min(best, s_min[k]);
-      if (best != kNone) atomicMin(&ctl->minsup, best);
-    }
-    if (last_block(&ctl->ticket) && threadIdx.x == 0) {
-      const uint32_t ms = ctl->minsup;
-      const unsigned long long alive = ctl->alive;
-      if (ms == kNone || alive == 0) {  // nothing live: the search is over
-        if ((ms == kNone) != (alive == 0)) ctl->err |= kErrPeelCount;
-        ctl->action = kRoundDone;
-        return;
-      }
-      ctl->level_kmax = ms + 2;
-      ctl->level_count = alive;
-      ctl->level_floor = ctl->thr;
-      ctl->levels += 1;
-      if (ctl->first && ctl->early_ans && alive == ctl->early_ans && (ctl->exact_early || ms + 2 == ctl->early_k)) {
-        ctl->level_kmax = ctl->early_k;
-        ctl->action = kRoundDone;
-        return;
-      }
-      ctl->first = 0;
-      if (ctl->max_levels && ctl->levels >= ctl->max_levels) {  // level budget spent
-        ctl->action = kRoundDone;
-        return;
-      }
-      ctl->thr = int32_t(ms) + 1;
-      ctl->action = (!rule.dense && 2ull * alive <= rule.mc) ? kRoundCompact : kRoundSweep;
-    }
+    level_end(ctl, state, sup, rule, [] {});
     return;
   }
   if (ctl->stamped || (a != kRoundPeel && a != kRoundDrop && a != kRoundEmpty)) return;
@@ -134,16 +164,8 @@ __global__ __launch_bounds__(kBlock) void k_round_stamp(uint32_t* q0, uint32_t* 
   const uint32_t thr = uint32_t(ctl->thr);
   const uint32_t* q = cur ? q1 : q0;
   const bool fb = fbits && a == kRoundPeel;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-    const uint32_t e = q[i];
-    state[e] = r;
-    del_thr[orig[e]] = thr;
-    if (fb) {
-      const uint32_t u = src[e], v = dst[e];
-      atomicOr(&fbits[uint64_t(u) * ws + (v >> 6)], 1ull << (v & 63));
-      atomicOr(&fbits[uint64_t(v) * ws + (u >> 6)], 1ull << (u & 63));
-    }
-  }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+    stamp_edge(q[i], r, thr, state, orig, del_thr, src, dst, fb ? fbits : nullptr, ws);
   if (last_block(&ctl->ticket) && threadIdx.x == 0) {
     ctl->alive -= cnt;
     ctl->rounds_done += 1;
@@ -176,26 +198,31 @@ __device__ __forceinline__ void clear_edge_bits(uint32_t u, uint32_t v, unsigned
   }
 }
 
-// Close a walked round: clear its frontier from the bit matrices, then (last
-// workgroup) make the next queue current and decide what the next round does.
-// After a sweep, only the decision on the swept frontier is taken.
-__global__ __launch_bounds__(kBlock) void k_round_advance(uint32_t* q0, uint32_t* q1, Ctl* ctl,
-                                                          const uint32_t* __restrict__ src,
-                                                          const uint32_t* __restrict__ dst,
-                                                          unsigned long long* bits, unsigned long long* fbits,
-                                                          uint32_t ws, RoundRule rule) {
+// Close a round. After a walked round: clear its frontier from the bit
+// matrices; the next frontier is the peel's next queue. After a sweep: the
+// next frontier is the swept queue. Every workgroup reads the same frontier
+// size, live count and threshold, so all take the same decision on the next
+// round and stamp the frontier it walks, drops or empties the level with (a
+// compaction takes it unstamped); an empty frontier completes the level
+// (level_end). The last workgroup switches queues after a walked round and
+// publishes the decision.
+__global__ __launch_bounds__(kBlock) void k_round_next(uint32_t* q0, uint32_t* q1, Ctl* ctl, uint32_t* state,
+                                                       const int32_t* __restrict__ sup,
+                                                       const uint32_t* __restrict__ orig, uint32_t* del_thr,
+                                                       const uint32_t* __restrict__ src,
+                                                       const uint32_t* __restrict__ dst, unsigned long long* bits,
+                                                       unsigned long long* fbits, uint32_t ws, RoundRule rule) {
   const uint32_t a = ctl->action;
-  if (a == kRoundSweep) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      const uint32_t c = ctl->qcount[ctl->cur];
-      ctl->stamped = 0;
-      set_action(ctl, decide_round(c, ctl->alive, ctl->thr, rule, false));
-    }
+  if (a == kRoundMinSup) {
+    level_end(ctl, state, sup, rule, [] {});
     return;
   }
-  if (a != kRoundPeel || !ctl->stamped) return;
-  const uint32_t cur = ctl->cur;
-  if (bits) {
+  const bool walked = a == kRoundPeel && ctl->stamped;
+  if (!walked && a != kRoundSweep) return;
+  const uint32_t cur = ctl->cur, r = ctl->round;
+  const int32_t thr = ctl->thr;
+  const unsigned long long alive = ctl->alive;
+  if (walked && bits) {
     const uint32_t cnt = ctl->qcount[cur];
     const uint32_t* q = cur ? q1 : q0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
@@ -203,14 +230,36 @@ __global__ __launch_bounds__(kBlock) void k_round_advance(uint32_t* q0, uint32_t
       clear_edge_bits(src[e], dst[e], bits, fbits, ws);
     }
   }
+  const uint32_t nq = walked ? cur ^ 1u : cur;  // queue of the next frontier
+  const uint32_t c = ctl->qcount[nq];
+  auto switch_queue = [&] {
+    if (walked) {
+      ctl->qcount[cur] = 0;
+      ctl->cur = nq;
+      ctl->round = r + 1;
+    }
+  };
+  if (c == 0) {
+    level_end(ctl, state, sup, rule, switch_queue);
+    return;
+  }
+  const uint32_t d = decide_round(c, alive, thr, rule, walked && !rule.dense);
+  const bool stamp = d == kRoundPeel || d == kRoundDrop || d == kRoundEmpty;
+  if (stamp) {
+    const uint32_t rs = walked ? r + 1 : r;
+    const uint32_t* q = nq ? q1 : q0;
+    unsigned long long* fb = d == kRoundPeel ? fbits : nullptr;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
+      stamp_edge(q[i], rs, uint32_t(thr), state, orig, del_thr, src, dst, fb, ws);
+  }
   if (last_block(&ctl->ticket) && threadIdx.x == 0) {
-    const uint32_t nxt = cur ^ 1u;
-    const uint32_t c = ctl->qcount[nxt];
-    ctl->qcount[cur] = 0;
-    ctl->cur = nxt;
-    ctl->round += 1;
-    ctl->stamped = 0;
-    set_action(ctl, decide_round(c, ctl->alive, ctl->thr, rule, !rule.dense));
+    switch_queue();
+    if (stamp) {
+      ctl->alive = alive - c;
+      ctl->rounds_done += 1;
+    }
+    ctl->stamped = stamp ? 1u : 0u;
+    ctl->action = d;
   }
 }
 
@@ -500,7 +549,7 @@ void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, 
   auto fbits = reinterpret_cast<unsigned long long*>(a.fbits);
   auto b0 = reinterpret_cast<const unsigned long long*>(a.bits0);
   const int peel_grid = work_grid(a.mc, kBlock, kPeelGrid);
-  const int advance_grid = rule.dense ? work_grid(a.mc, 4 * kBlock, kAdvanceGrid) : 1;
+  const int next_grid = work_grid(a.mc, 4 * kBlock, kStampGrid);
   for (int i = 0; i < count; ++i) {
     if (rule.dense) {
       // every argument as an lvalue for the event-stamped launch
@@ -534,10 +583,9 @@ void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, 
       }
 #undef KT_SPARSE_ROUND
     }
-    k_round_advance<<<advance_grid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.src, a.dst,
-                                                                     rule.dense ? bits : nullptr,
-                                                                     rule.dense ? fbits : nullptr, a.ws, rule);
-    launch_stamp(a, ctl, rule, s);
+    k_round_next<<<next_grid, kBlock, 0, s>>>(a.q0, a.q1, ctl, a.state, a.sup, a.orig, a.del_thr, a.src, a.dst,
+                                              rule.dense ? bits : nullptr, rule.dense ? fbits : nullptr, a.ws,
+                                              rule);
   }
 }
 

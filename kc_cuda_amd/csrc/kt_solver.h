@@ -287,6 +287,7 @@ class Solver {
   hipStream_t stream_ = nullptr;
   Ctl* ctl_ = nullptr;
   uint32_t* pinned_ = nullptr;
+  uint32_t* pinned_hist_ = nullptr;  // kDegBins words of pinned memory (none in arena mode)
   uint32_t host_rb_[64] = {};  // readback buffer in arena mode
   hipEvent_t ev_[10];
   int lanes_count_ = 0, lanes_peel_ = 0;
@@ -307,6 +308,7 @@ class Solver {
   bool gallop_ = true;
   uint32_t count_kind_ = 0, count_words_ = 0;  // the last timed support count
   uint32_t walk_levels_ = 16;
+  uint32_t build_tiles_ = 4096;  // most tiles of the partitioned CSR build
   int probe_push_ = 0;               // measurement of a kernel (mode bits)
   int probe_kernel_ = kProbeVertexPeel;
   hipEvent_t* probe_events(int launches);  // 2 * launches events from the pool

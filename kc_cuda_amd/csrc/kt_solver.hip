@@ -68,7 +68,10 @@ Solver::Solver(int device) : device_(device) {
   KT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   KT_HIP(hipMalloc(&ctl_, sizeof(Ctl)));
   KT_HIP(hipMemsetAsync(ctl_, 0, sizeof(Ctl), stream_));
-  KT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 256, hipHostMallocDefault));
+  // 256 B of control-block and counter readbacks, then the degree histogram
+  KT_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), 256 + sizeof(uint32_t) * kDegBins,
+                       hipHostMallocDefault));
+  pinned_hist_ = pinned_ + 64;
   init_common();
 }
 
@@ -119,6 +122,7 @@ void Solver::init_common() {
   if (const char* v = std::getenv("KT_TRACE")) trace_ = v[0] == '1';
   if (const char* v = std::getenv("KT_EIDM_BUDGET_MB")) eidm_budget_ = uint64_t(std::atoll(v)) << 20;
   if (const char* v = std::getenv("KT_WALK_LEVELS")) walk_levels_ = std::max(1, std::atoi(v));
+  if (const char* v = std::getenv("KT_BUILD_TILES")) build_tiles_ = uint32_t(std::max(1, std::atoi(v)));
   if (const char* v = std::getenv("KT_VERTEX_PEEL"))
     vertex_mode_ = std::strcmp(v, "rounds") == 0 ? 1 : std::strcmp(v, "cascade") == 0 ? 2 : 0;
   default_drop_ratio_ = sparse_drop_ratio_;
@@ -238,21 +242,28 @@ bool Solver::build_partitioned(int mode, const uint32_t* a, const uint32_t* b, u
   uint32_t S = 0, B = 0;
   if (!csr_partition_plan(n, &S, &B)) return false;
   const uint64_t entries = mode == 0 ? 2 * items : items;
-  const uint64_t tile = std::max<uint64_t>(65536, (items + 4095) / 4096);
+  // at most build_tiles_ tiles of at least 65536 items
+  const uint64_t tiles = std::max<uint32_t>(1, build_tiles_);
+  const uint64_t tile = std::max<uint64_t>(65536, (items + tiles - 1) / tiles);
   const uint32_t T = uint32_t(std::max<uint64_t>(1, (items + tile - 1) / tile));
   const uint64_t bt = uint64_t(B) * T;
-  hist_.ensure(sizeof(uint32_t) * (bt + 1));
-  hoff_.ensure(sizeof(uint64_t) * (bt + 1));
+  hist_.ensure(sizeof(uint32_t) * (2 * bt + 1));   // tile-major counts, then bucket-major
+  hoff_.ensure(sizeof(uint64_t) * (2 * bt + 1));   // bucket-major offsets, then tile-major
   part_.ensure(sizeof(uint32_t) * 2 * std::max<uint64_t>(entries, 1));
-  fill_u32(hist_.as<uint32_t>() + bt, 1, 0, stream_);
+  uint32_t* hist_t = hist_.as<uint32_t>();
+  uint32_t* hist_b = hist_t + bt;
+  uint64_t* hoff_b = hoff_.as<uint64_t>();
+  uint64_t* hoff_t = hoff_b + bt + 1;
+  fill_u32(hist_b + bt, 1, 0, stream_);
   zero_u32(&ctl_->err);
-  csr_part_hist(mode, a, b, items, tile, n, S, B, T, hist_.as<uint32_t>(), &ctl_->err, stream_);
+  csr_part_hist(mode, a, b, items, tile, n, S, B, T, hist_t, &ctl_->err, stream_);
   if (read_u32(&ctl_->err)) fail("edge list holds a vertex id >= n");
-  exclusive_scan_u32_to_u64(hist_.as<uint32_t>(), hoff_.as<uint64_t>(), uint32_t(bt + 1));
-  csr_part_scatter(mode, a, b, items, tile, n, S, B, T, hoff_.as<uint64_t>(), part_.as<uint32_t>(), stream_);
-  csr_bucket_rows(part_.as<uint32_t>(), hoff_.as<uint64_t>(), B, T, S, n, off(), deg(), adj(), stream_);
-  KT_HIP(hipMemcpyAsync(off() + n, hoff_.as<uint64_t>() + bt, sizeof(uint64_t), hipMemcpyDeviceToDevice,
-                        stream_));
+  transpose_u32(hist_t, T, B, hist_b, stream_);
+  exclusive_scan_u32_to_u64(hist_b, hoff_b, uint32_t(bt + 1));
+  transpose_u64(hoff_b, B, T, hoff_t, stream_);
+  csr_part_scatter(mode, a, b, items, tile, n, S, B, T, hoff_t, part_.as<uint32_t>(), stream_);
+  csr_bucket_rows(part_.as<uint32_t>(), hoff_b, B, T, S, n, off(), deg(), adj(), stream_);
+  KT_HIP(hipMemcpyAsync(off() + n, hoff_b + bt, sizeof(uint64_t), hipMemcpyDeviceToDevice, stream_));
   return true;
 }
 

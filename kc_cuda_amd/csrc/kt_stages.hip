@@ -205,8 +205,10 @@ bool Solver::filtered_start(LevelGraph** gp, uint32_t* k_out, uint64_t* alive_ou
   if (n < 4 * kMinCut) return false;
   dhist_.ensure(sizeof(uint32_t) * kDegBins);
   degree_hist(g0.deg, n, dhist_.as<uint32_t>(), stream_);
-  std::vector<uint32_t> h(kDegBins);
-  KT_HIP(hipMemcpyAsync(h.data(), dhist_.p, sizeof(uint32_t) * kDegBins, hipMemcpyDeviceToHost, stream_));
+  // into pinned host memory (arena mode has none and copies to pageable memory)
+  std::vector<uint32_t> paged(pinned_hist_ ? 0 : kDegBins);
+  uint32_t* h = pinned_hist_ ? pinned_hist_ : paged.data();
+  KT_HIP(hipMemcpyAsync(h, dhist_.p, sizeof(uint32_t) * kDegBins, hipMemcpyDeviceToHost, stream_));
   KT_HIP(hipStreamSynchronize(stream_));
   // deg_at_least_[b]: vertices whose degree is at least the lower edge of bin b
   deg_at_least_.assign(kDegBins + 1, 0);
