@@ -160,8 +160,7 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
  * for every pair in its graph's kmax-truss. The valid graphs are solved
  * together as one disjoint union: one truss decomposition of the union on
  * this context, from which every graph's answer is read. Results equal
- * kt_solve_pairs_* on each graph. KT_BATCH_GRAPH is accepted and ignored. */
-#define KT_BATCH_GRAPH 1
+ * kt_solve_pairs_* on each graph. flags is reserved and must be 0. */
 int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
                    const uint32_t* h_ns, uint32_t count, uint32_t* d_kmax, uint64_t* d_count,
                    uint32_t* d_status, uint8_t* d_keep, int flags);

@@ -362,15 +362,14 @@ class Context:
     def copy_device(self, dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
         _check(self._lib.kt_memcpy_d2d(self._h, dst.ptr, src.ptr, nbytes))
 
-    def batch_solve(self, dpairs: DeviceArray, offsets: np.ndarray, ns: np.ndarray, keep: bool = False,
-                    graph: bool = False) -> "BatchResult":
+    def batch_solve(self, dpairs: DeviceArray, offsets: np.ndarray, ns: np.ndarray,
+                    keep: bool = False) -> "BatchResult":
         """Solve every graph of a batch (pairs already in HBM, canonical and
         sorted within each graph; offsets: count + 1 pair offsets; ns: vertex
-        counts): the valid graphs are solved together as one disjoint union.
-        graph is accepted for compatibility and has no effect."""
+        counts): the valid graphs are solved together as one disjoint union."""
         run = self.batch(dpairs, offsets, ns, keep)
         try:
-            run.solve(graph)
+            run.solve()
             return run.result()
         finally:
             run.free()
@@ -453,12 +452,11 @@ class BatchRun(_BatchBuffers):
         super().__init__(ctx, offsets, ns, keep)
         self.dpairs = dpairs
 
-    def solve(self, graph: bool = False) -> None:
+    def solve(self) -> None:
         lib = self.ctx._lib
         _check(lib.kt_batch_solve(self.ctx._h, self.dpairs.ptr, self.offsets.ctypes.data,
                                   self.ns.ctypes.data, self.count, self.kmax.ptr, self.edges.ptr,
-                                  self.status.ptr, self.keep.ptr if self.keep else None,
-                                  1 if graph else 0))
+                                  self.status.ptr, self.keep.ptr if self.keep else None, 0))
 
     def result(self) -> BatchResult:
         return self.fetch()

@@ -277,9 +277,9 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
 int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets, const uint32_t* h_ns,
                    uint32_t count, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status, uint8_t* d_keep,
                    int flags) {
-  (void)flags;  // KT_BATCH_GRAPH: the union solve has host decisions and is never captured
   if (!ctx || (count && (!d_pairs || !h_offsets || !h_ns || !d_kmax || !d_count || !d_status)))
     return set_err("kt_batch_solve: NULL argument");
+  if (flags != 0) return set_err("kt_batch_solve: flags is reserved and must be 0");
   return guarded([&] { ctx->batcher().solve(d_pairs, h_offsets, h_ns, count, d_kmax, d_count, d_status, d_keep); });
 }
 

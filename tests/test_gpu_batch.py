@@ -92,8 +92,8 @@ def test_batch_flags_invalid_graphs(ctx):
 
 
 def test_batch_repeat_and_autotune(ctx):
-    """Repeated solves (with the compatibility graph flag, and with other solves
-    on the context in between) return the same answers; the autotuner's
+    """Repeated solves (with other solves on the context in between) return
+    the same answers, non-zero flags are rejected; the autotuner's
     trials agree, its final solve of the whole batch equals the untuned one,
     and so does a solve with the settings it leaves on the context."""
     from kc_cuda_amd import gen
@@ -103,14 +103,21 @@ def test_batch_repeat_and_autotune(ctx):
         ref = ctx.batch_solve(d, offs, ns, keep=True)
         _check_against_plain(ctx, pairs, offs, ns, ref)
         for _ in range(2):
-            r = ctx.batch_solve(d, offs, ns, keep=True, graph=True)
+            r = ctx.batch_solve(d, offs, ns, keep=True)
             np.testing.assert_array_equal(r.kmax, ref.kmax)
             np.testing.assert_array_equal(r.edges, ref.edges)
             np.testing.assert_array_equal(r.keep, ref.keep)
             ctx.solve_pairs(pairs[:int(offs[1])], int(ns[0]))
-        from kc_cuda_amd import KtError
+        from kc_cuda_amd import KtError, _check
         with pytest.raises(KtError, match="sample_every"):
             ctx.batch_autotune(d, offs, ns, sample_every=0)
+        run = ctx.batch(d, offs, ns, False)
+        try:
+            with pytest.raises(KtError, match="reserved"):
+                _check(ctx._lib.kt_batch_solve(ctx._h, d.ptr, run.offsets.ctypes.data, run.ns.ctypes.data,
+                                               run.count, run.kmax.ptr, run.edges.ptr, run.status.ptr, None, 1))
+        finally:
+            run.free()
         try:
             for every in (3, 1):
                 r, t = ctx.batch_autotune(d, offs, ns, sample_every=every)
