@@ -66,6 +66,16 @@ void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t
 // lvl[v] = (lvl[v] > k) ? kNone : (lvl[v] == kNone ? kNone : k) -- used to
 // re-open every vertex whose removal threshold exceeds k.
 void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s);
+// Max core of a level of at most core_decompose_max_n() vertices in one
+// workgroup, from its current state (the alive vertices, their alive-
+// neighbour degrees, every alive degree >= k0 - 1): thresholds walked upward
+// from k0 without restores; afterwards the max core is alive with its
+// degrees, every other vertex v of the level removed here has lvl[v] = its
+// core number + 1, out[0] = the max core's k (0 if nothing was alive) and
+// out[1] its vertex count. q holds n entries.
+uint32_t core_decompose_max_n();
+void core_decompose(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t k0, uint32_t* lvl, uint32_t* deg,
+                    uint32_t* q, uint32_t* out, hipStream_t s);
 // Degree histogram (kDegBins bins: exact below kDegExact, then one per power
 // of two) and the degree filter of the max-core search (lvl = k for every
 // vertex of degree < k, kNone otherwise; *alive += vertices kept).

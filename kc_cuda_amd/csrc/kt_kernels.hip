@@ -756,6 +756,140 @@ __global__ void k_core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k) {
   }
 }
 
+// -------------------------------------------- small-level core numbers ---
+
+// The max-core search on a small level in one workgroup: the degrees of the
+// level's vertices live in LDS, and thresholds are walked upward from the
+// current one without restoring anything (k-cores are nested), each jumping
+// to the minimum alive degree + 1. A threshold queues every alive vertex of
+// degree < k (lvl = k), and the queue is drained in steps separated by
+// barriers, one wave per queued vertex: every neighbour loses one degree, and
+// the decrement that takes it from k to k - 1 queues it (an LDS atomic, so
+// no vertex is queued twice). When a threshold removes every alive vertex,
+// the vertices it removed are the max core: they are re-opened (lvl = kNone)
+// and their degrees recounted among themselves. A removed vertex's LDS
+// degree carries bit 31; afterwards lvl[v] = 1 + the core number of every
+// vertex removed here, so every threshold from k0 up is an exact cut.
+constexpr uint32_t kDecompBlock = 1024;
+constexpr uint32_t kDecompMaxN = 16000;   // LDS degrees: 64,000 B
+constexpr uint32_t kDecompUnroll = 8;     // row slots a lane loads before its atomics
+constexpr uint32_t kGone = 0x80000000u;
+constexpr uint32_t kCoreMark = 0x40000000u;
+__global__ __launch_bounds__(kDecompBlock) void k_core_decompose(const uint64_t* __restrict__ off,
+                                                                  const uint32_t* __restrict__ adj, uint32_t n,
+                                                                  uint32_t k0, uint32_t* lvl, uint32_t* deg,
+                                                                  uint32_t* q, uint32_t* out) {
+  extern __shared__ uint32_t s_deg[];
+  __shared__ uint32_t s_tail, s_alive, s_min;
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+  constexpr uint32_t kWaves = kDecompBlock / kWave;
+  if (tid == 0) {
+    s_tail = 0;
+    s_alive = 0;
+    s_min = kNone;
+  }
+  __syncthreads();
+  for (uint32_t v = tid; v < n; v += kDecompBlock) {
+    const bool alive = lvl[v] == kNone;
+    s_deg[v] = alive ? deg[v] : kGone;
+    if (alive) {
+      atomicAdd(&s_alive, 1u);
+      atomicMin(&s_min, deg[v]);
+    }
+  }
+  __syncthreads();
+  uint32_t k = max(k0, s_min + 1);  // every alive degree is >= s_min: the first removing threshold
+  uint32_t start = 0;               // queue position where threshold k's removals begin
+  bool done = s_alive == 0;
+  while (!done) {
+    // the threshold's first vertices
+    for (uint32_t v = tid; v < n; v += kDecompBlock) {
+      const uint32_t d = s_deg[v];
+      if (d < k) {  // alive (bit 31 clear) with degree < k
+        s_deg[v] = d | kGone;
+        lvl[v] = k;
+        q[atomicAdd(&s_tail, 1u)] = v;
+      }
+    }
+    __syncthreads();
+    // drain: one wave per queued vertex, in steps separated by barriers
+    uint32_t head = start;
+    for (;;) {
+      const uint32_t tail = s_tail;
+      if (head == tail) break;
+      for (uint32_t i = head + wid; i < tail; i += kWaves) {
+        const uint32_t v = q[i];
+        const uint64_t b = off[v], e = off[v + 1];
+        for (uint64_t j0 = b; j0 < e; j0 += kWave * kDecompUnroll) {
+          uint32_t w[kDecompUnroll];  // independent loads in flight before the atomics
+#pragma unroll
+          for (uint32_t u = 0; u < kDecompUnroll; ++u) {
+            const uint64_t j = j0 + u * kWave + lane;
+            w[u] = j < e ? adj[j] : kNone;
+          }
+#pragma unroll
+          for (uint32_t u = 0; u < kDecompUnroll; ++u)
+            if (w[u] != kNone && atomicSub(&s_deg[w[u]], 1u) == k) {  // w falls below k now
+              atomicOr(&s_deg[w[u]], kGone);
+              lvl[w[u]] = k;
+              q[atomicAdd(&s_tail, 1u)] = w[u];
+            }
+        }
+      }
+      head = tail;
+      __syncthreads();
+    }
+    // s_tail is final for this threshold
+    const uint32_t removed = s_tail - start;
+    __syncthreads();
+    if (tid == 0) {
+      s_alive -= removed;
+      s_min = kNone;
+    }
+    __syncthreads();
+    if (s_alive == 0) {
+      done = true;
+    } else {
+      for (uint32_t v = tid; v < n; v += kDecompBlock) {
+        const uint32_t d = s_deg[v];
+        if (!(d & kGone)) atomicMin(&s_min, d);
+      }
+      __syncthreads();
+      start = s_tail;
+      k = s_min + 1;
+      __syncthreads();
+    }
+  }
+  // the last threshold's removals (queue [start, s_tail)) are the max core
+  const uint32_t tail = s_tail;
+  for (uint32_t i = start + tid; i < tail; i += kDecompBlock) s_deg[q[i]] = kCoreMark;
+  __syncthreads();
+  for (uint32_t i = start + wid; i < tail; i += kWaves) {
+    const uint32_t v = q[i];
+    const uint64_t b = off[v], e = off[v + 1];
+    uint32_t c = 0;
+    for (uint64_t j0 = b; j0 < e; j0 += kWave * kDecompUnroll) {
+      uint32_t w[kDecompUnroll];
+#pragma unroll
+      for (uint32_t u = 0; u < kDecompUnroll; ++u) {
+        const uint64_t j = j0 + u * kWave + lane;
+        w[u] = j < e ? adj[j] : kNone;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kDecompUnroll; ++u) c += w[u] != kNone && s_deg[w[u]] == kCoreMark;
+    }
+    c = wave_sum(c);
+    if (lane == 0) {
+      deg[v] = c;
+      lvl[v] = kNone;
+    }
+  }
+  if (tid == 0) {
+    out[0] = tail > start ? k - 1 : 0;  // the max core's k (0: nothing was alive)
+    out[1] = tail - start;              // its vertices
+  }
+}
+
 // --------------------------------------------------- (a1-a3) extraction ---
 
 // key = degree-in-core << 32 | (~id): ascending order = cmp_link's
@@ -1694,6 +1828,13 @@ void core_filter(const uint32_t* deg, uint32_t n, uint32_t k, uint32_t* lvl, uin
 }
 void core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k, hipStream_t s) {
   if (n) k_core_reopen_above<<<grid_for(n), kBlock, 0, s>>>(lvl, n, k);
+}
+uint32_t core_decompose_max_n() { return kDecompMaxN; }
+void core_decompose(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t k0, uint32_t* lvl, uint32_t* deg,
+                    uint32_t* q, uint32_t* out, hipStream_t s) {
+  if (n > kDecompMaxN) fail("core_decompose: level too large for the LDS degrees");
+  fill_u32(out, 2, 0, s);
+  if (n) k_core_decompose<<<1, kDecompBlock, sizeof(uint32_t) * n, s>>>(off, adj, n, k0, lvl, deg, q, out);
 }
 
 void core_collect(const uint32_t* lvl, const uint32_t* deg, const uint32_t* to_orig, uint32_t n,

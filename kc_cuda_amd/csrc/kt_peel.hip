@@ -19,6 +19,7 @@
 // kernel whose action does not concern it does nothing. k_round_stamp stamps
 // the first frontier of a search (or ends the level, when it is empty).
 #include <algorithm>
+#include <cstdlib>
 
 #include "kt_device.h"
 #include "kt_kernels.h"
@@ -548,8 +549,12 @@ void peel_rounds(const PeelArgs& a, Ctl* ctl, const RoundRule& rule, int lanes, 
   auto bits = reinterpret_cast<unsigned long long*>(a.bits);
   auto fbits = reinterpret_cast<unsigned long long*>(a.fbits);
   auto b0 = reinterpret_cast<const unsigned long long*>(a.bits0);
+  static const int next_cap = [] {  // tuning only
+    const char* v = std::getenv("KT_NEXT_GRID");
+    return v ? std::max(1, std::atoi(v)) : kStampGrid;
+  }();
   const int peel_grid = work_grid(a.mc, kBlock, kPeelGrid);
-  const int next_grid = work_grid(a.mc, 4 * kBlock, kStampGrid);
+  const int next_grid = work_grid(a.mc, 4 * kBlock, next_cap);
   for (int i = 0; i < count; ++i) {
     if (rule.dense) {
       // every argument as an lvalue for the event-stamped launch

@@ -55,6 +55,9 @@ struct LevelGraph {
   uint32_t* to_orig = nullptr;   // null on level 0
   uint32_t k_enter = 0;          // every vertex here has core number >= k_enter
   std::vector<uint32_t> ok;      // thresholds whose peel left vertices alive
+  // after a core decomposition: lvl is exact for thresholds in
+  // [exact_from, exact_to] (the t-core = alive or lvl > t); 0, 0 otherwise
+  uint32_t exact_from = 0, exact_to = 0;
   DevBuf b_off, b_adj, b_deg, b_lvl, b_orig;
 };
 
@@ -200,9 +203,17 @@ class Solver {
   // 2 always the cascade
   void set_vertex_mode(int m) { vertex_mode_ = m; }
   static constexpr uint64_t kCascadeMaxSlots = uint64_t(1) << 24;
+  // a level of at most core_decompose_max_n() vertices and this many row
+  // slots finishes the max-core search in one workgroup (decompose_level)
+  // when its rows average at most kDecomposeMaxRow slots or at most
+  // kDecomposeMaxRange thresholds are left
+  static constexpr uint64_t kDecomposeMaxSlots = uint64_t(1) << 25;
+  static constexpr uint32_t kDecomposeMaxRow = 2048;
+  static constexpr uint32_t kDecomposeMaxRange = 64;
   // fresh: start from full degrees with every vertex alive; otherwise from
   // the state already on level 0. allow_cut: continue on induced subgraphs.
   uint32_t max_core(bool allow_cut = true, bool fresh = true);
+  uint32_t decompose_level(LevelGraph& g, uint32_t k);
   bool filtered_start(LevelGraph** gp, uint32_t* k_out, uint64_t* alive_out);
   LevelGraph& level0();
   // recount: the alive vertices' degrees in `parent` are stale (a degree
@@ -294,7 +305,7 @@ class Solver {
   bool force_edge_count_ = false;
   bool force_atomic_build_ = false;
   int core_mode_ = 0;
-  DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_, dhist_, vq_;
+  DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_, dhist_, vq_, dec_out_;
   int vertex_mode_ = 0;        // see set_vertex_mode
   bool vertex_rounds_ = false;  // the last vertex peel ran synchronous rounds (reported by last_stats)
   std::vector<uint64_t> deg_at_least_;  // input degree histogram, suffix sums per bin (empty: none)

@@ -135,7 +135,7 @@ void Solver::each_buffer(F f) const {
                          &adj_d_, &src_, &din_, &din_off_, &tkeys_, &tvals_, &tiota_, &in_src_, &off_u_,
                          &adj_u_, &eid_u_, &sup_, &state_, &eq0_, &eq1_, &out_pairs_, &truss_out_, &del_thr_, &orig0_,
                          &temp_, &hist_, &hoff_, &part_, &bits0_, &bits_, &fbits_, &pref0_, &eidm_, &heavy_,
-                         &vflag_, &vmap_, &roff_, &dhist_, &vq_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_, &text_,
+                         &vflag_, &vmap_, &roff_, &dhist_, &vq_, &dec_out_, &bk_state_, &bk_sup_, &bk_bits_, &bk_del_, &text_,
                          &tile_cnt_, &tile_off_, &lx_, &ly_, &lkind_, &lx2_, &ly2_, &lkeep_, &lkeep_off_,
                          &lstat_, &probe_stats_};
   for (const DevBuf* b : all) f(*b);
@@ -348,6 +348,7 @@ LevelGraph& Solver::level0() {
   g.to_orig = nullptr;
   g.k_enter = 0;
   g.ok.clear();
+  g.exact_from = g.exact_to = 0;
   return g;
 }
 

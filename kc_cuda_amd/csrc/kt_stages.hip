@@ -140,6 +140,7 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive, b
   s.lvl = s.b_lvl.as<uint32_t>();
   s.k_enter = k_enter;
   s.ok.assign(1, k_enter);
+  s.exact_from = s.exact_to = 0;
   fill_u32(s.deg + na, 1, 0, stream_);
   sub_degrees(list_.as<uint32_t>(), recount ? nullptr : p.deg, p.to_orig, na, s.deg, s.to_orig, stream_);
   if (recount) list_alive_degrees(p.off, p.adj, list_.as<uint32_t>(), na, abits, acoarse, s.deg, heavy(), stream_);
@@ -262,10 +263,21 @@ uint32_t Solver::max_core(bool allow_cut, bool fresh) {
     if (gone >= before) return 0;
     alive = before - gone;
     g->ok.assign(1, 1);
+    g->exact_from = g->exact_to = 0;
     if (allow_cut && alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, 1, uint32_t(alive));
   }
   bool grow = true;
+  uint32_t hi = ~0u;  // smallest threshold known to empty the level
   while (step > 0) {
+    // A small level finishes in one workgroup that walks every threshold
+    // (each costs about a row length of latency), the gallop in cascades of
+    // the whole level (about log2 of the range): walk when few thresholds
+    // are left, or on a cut level (its threshold already near the max core)
+    // whose rows are short.
+    if (vertex_mode_ == 0 && g->n <= core_decompose_max_n() && g->slots <= kDecomposeMaxSlots &&
+        (uint64_t(hi) - k <= kDecomposeMaxRange ||
+         (depth_ > 1 && g->slots <= uint64_t(kDecomposeMaxRow) * g->n)))
+      return decompose_level(*g, k);
     const uint32_t kk = k + step;
     KT_HIP(hipMemcpyAsync(deg_bak_.p, g->deg, sizeof(uint32_t) * g->n, hipMemcpyDeviceToDevice, stream_));
     const uint64_t removed = peel_vertices(*g, kk);
@@ -277,6 +289,7 @@ uint32_t Solver::max_core(bool allow_cut, bool fresh) {
       if (allow_cut && alive * 4 <= g->n && alive >= kMinCut) g = &cut_level(*g, k, uint32_t(alive));
     } else {
       // overshoot: re-open what this step removed and restore degrees
+      hi = std::min(hi, kk);
       core_reopen_above(g->lvl, g->n, k, stream_);
       KT_HIP(hipMemcpyAsync(g->deg, deg_bak_.p, sizeof(uint32_t) * g->n, hipMemcpyDeviceToDevice, stream_));
       if (step == 1) break;
@@ -287,9 +300,33 @@ uint32_t Solver::max_core(bool allow_cut, bool fresh) {
   return k;
 }
 
+// The rest of the max-core search on a small level in one workgroup
+// (core_decompose): from the k-core alive on g, thresholds walk upward with
+// no restores. Leaves the max core alive on g with its degrees and every
+// vertex removed on the way stamped with its core number + 1, so every cut
+// of g from k up to the max core is exact; returns the max core's k.
+uint32_t Solver::decompose_level(LevelGraph& g, uint32_t k) {
+  dec_out_.ensure(sizeof(uint32_t) * 2);
+  q0_.ensure(sizeof(uint32_t) * (size_t(g.n) + 1));
+  if (trace_) trace_ms();
+  core_decompose(g.off, g.adj, g.n, k, g.lvl, g.deg, q0_.as<uint32_t>(), dec_out_.as<uint32_t>(), stream_);
+  KT_HIP(hipMemcpyAsync(pinned_, dec_out_.p, sizeof(uint32_t) * 2, hipMemcpyDeviceToHost, stream_));
+  KT_HIP(hipStreamSynchronize(stream_));
+  const uint32_t kmax = pinned_[0], alive = pinned_[1];
+  if (kmax < k || alive == 0 || alive > g.n) fail("vertex peel bookkeeping mismatch (core decomposition)");
+  if (trace_)
+    std::fprintf(stderr, "[kt] core decomposition level_n=%u slots=%llu from k=%u: max core %u, %u vertices (%.2f ms)\n",
+                 g.n, (unsigned long long)g.slots, k, kmax, alive, trace_ms());
+  g.ok.push_back(kmax);
+  g.exact_from = k;
+  g.exact_to = kmax;
+  return kmax;
+}
+
 // Leaves the k-core alive on the deepest level whose vertices all have core
 // number >= its entry threshold <= k: re-open everything that level removed
-// above its largest successful threshold kj <= k, recount degrees, peel at k.
+// above its largest successful threshold kj <= k (k itself when the level's
+// stamps are exact there), recount degrees, peel at k.
 void Solver::core_at(uint32_t k) {
   size_t d = depth_ - 1;
   while (d > 0 && levels_[d]->k_enter > k) --d;
@@ -306,9 +343,16 @@ void Solver::core_at(uint32_t k) {
   uint32_t kj = std::max<uint32_t>(g.k_enter, 1);
   for (uint32_t t : g.ok)
     if (t <= k) kj = std::max(kj, t);
+  const bool exact = g.exact_from && g.exact_from <= k && k <= g.exact_to;
+  if (exact) kj = k;
   core_reopen_above(g.lvl, g.n, kj, stream_);
   core_alive_degrees(g.off, g.adj, g.lvl, g.n, g.deg, heavy(), stream_);
   if (k > kj) peel_vertices(g, k);
+  // stamps at or below k are untouched; above k they are gone
+  if (exact)
+    g.exact_to = k;
+  else
+    g.exact_from = g.exact_to = 0;
 }
 
 // --------------------------------------------------- (a1-a3) extraction ---
