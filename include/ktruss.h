@@ -138,7 +138,9 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
  * after the first level, KT_FORCE_VERTEX_ROUNDS / KT_FORCE_VERTEX_CASCADE
  * the vertex peel by synchronous rounds / by one cascade launch per
  * threshold at every level (default: the cascade on levels of up to 2^24
- * row slots, rounds above). Every variant computes the same result. The
+ * row slots, rounds above, and the max-core search finished in one
+ * workgroup on levels of up to 16,000 vertices, which either flag turns
+ * off). Every variant computes the same result. The
  * sparse-peel drop ratio returns to its default (KT_SPARSE_DROP_RATIO or
  * 3). */
 #define KT_FORCE_EDGE_COUNT 1
