@@ -510,6 +510,7 @@ __global__ __launch_bounds__(kBlock) void k_core_push(const uint64_t* __restrict
 // past the capacity or a wait past kSpinLimit sets vq_err, which makes every
 // wave leave too (the host then fails the solve).
 constexpr uint64_t kCascadeChunk = 4096;
+constexpr uint32_t kCascadeUnroll = 4;  // groups of 64 row slots per wave step
 constexpr unsigned long long kItemEmpty = ~0ull;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
@@ -636,28 +637,37 @@ __global__ __launch_bounds__(kBlock) void k_core_cascade(const uint64_t* __restr
     }
     const uint64_t b = rb + uint64_t(c) * kCascadeChunk;
     const uint64_t e = min(re, b + kCascadeChunk);
-    for (uint64_t j0 = b; j0 < e; j0 += kWave) {
-      const uint64_t j = j0 + uint64_t(lane);
-      uint32_t w = 0;
-      bool cross = false;
-      if (j < e) {
-        w = adj[j];
-        cross = atomicSub(&deg[w], 1u) == k;
+    // kCascadeUnroll groups of 64 slots per step: their loads, then their
+    // returning decrements, are in flight together
+    for (uint64_t j0 = b; j0 < e; j0 += kWave * kCascadeUnroll) {
+      uint32_t w[kCascadeUnroll];
+      bool cross[kCascadeUnroll];
+#pragma unroll
+      for (uint32_t u = 0; u < kCascadeUnroll; ++u) {
+        const uint64_t j = j0 + u * kWave + uint64_t(lane);
+        w[u] = j < e ? adj[j] : 0u;
       }
-      const unsigned long long m = __ballot(cross);
-      if (m) {
-        const uint32_t leader = uni(uint32_t(__ffsll(m) - 1));
-        const uint32_t cnt = uni(uint32_t(__popcll(m)));
-        uint32_t base = 0;
-        if (uint32_t(lane) == leader) base = atomicAdd(&ctl->vq_reserve, cnt);
-        base = uni(__shfl(base, int(leader)));
-        if (uint64_t(base) + cnt > cap) {
-          if (lane == 0) atomicOr(&ctl->vq_err, kVqOverflow);
-          quit = 1;
-        } else if (cross) {
-          lvl[w] = k;
-          st_item(&q[base + uint32_t(__popcll(m & below))], static_cast<unsigned long long>(w));
+#pragma unroll
+      for (uint32_t u = 0; u < kCascadeUnroll; ++u)
+        cross[u] = j0 + u * kWave + uint64_t(lane) < e && atomicSub(&deg[w[u]], 1u) == k;
+#pragma unroll
+      for (uint32_t u = 0; u < kCascadeUnroll; ++u) {
+        const unsigned long long m = __ballot(cross[u]);
+        if (m) {
+          const uint32_t leader = uni(uint32_t(__ffsll(m) - 1));
+          const uint32_t cnt = uni(uint32_t(__popcll(m)));
+          uint32_t base = 0;
+          if (uint32_t(lane) == leader) base = atomicAdd(&ctl->vq_reserve, cnt);
+          base = uni(__shfl(base, int(leader)));
+          if (uint64_t(base) + cnt > cap) {
+            if (lane == 0) atomicOr(&ctl->vq_err, kVqOverflow);
+            quit = 1;
+          } else if (cross[u]) {
+            lvl[w[u]] = k;
+            st_item(&q[base + uint32_t(__popcll(m & below))], static_cast<unsigned long long>(w[u]));
+          }
         }
+        if (quit) break;
       }
       if (quit) break;
     }

@@ -32,6 +32,7 @@ namespace {
 // the edges, for a level's minimum support).
 constexpr int kStampGrid = 256;
 constexpr int kPeelGrid = 1024;
+constexpr uint32_t kPeelUnroll = 4;  // triangles per lane per step of the dense peel
 constexpr int kClearGrid = 512;
 
 // What the next round does with a frontier of c edges when `alive` edges are
@@ -362,35 +363,51 @@ __global__ __launch_bounds__(kBlock) void k_dense_peel_round(
     s_fb[threadIdx.x] = fb;
     s_inc[threadIdx.x] = inc;
     wave_sync();
-    for (uint32_t j0 = 0; j0 < total; j0 += kWave) {
-      const uint32_t j = j0 + uint32_t(lane);
-      uint32_t xa = kNone, xb = kNone;  // partner edges (u, w) and (v, w) to decrement
-      if (j < total) {
-        uint32_t lo = 0, hi = kWave - 1;  // owner: first lane whose inclusive count exceeds j
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_inc[wbase + mid] > j)
-            hi = mid;
-          else
-            lo = mid + 1;
-        }
-        const unsigned long long tw = s_t[wbase + lo];
-        const uint32_t b = select_bit(tw, j - (s_inc[wbase + lo] - uint32_t(__popcll(tw))));
-        const uint32_t w = ((ch * cw + lo) << 6) + b;
-        const bool ina = (s_fa[wbase + lo] >> b) & 1ull, inb = (s_fb[wbase + lo] >> b) & 1ull;
-        if (!ina && !inb) {
-          xa = eid_of(u, w);
-          xb = eid_of(v, w);
-        } else if (ina) {  // (u, w) in the frontier, third edge (v, w)
-          if (w < v) xb = eid_of(v, w);
-        } else {  // (v, w) in the frontier, third edge (u, w)
-          if (w < u) xa = eid_of(u, w);
+    // kPeelUnroll triangles per lane per step: their partner lookups, then
+    // all their decrements, are in flight together (the returning atomics
+    // are performed past the L2, so one round trip per triangle would leave
+    // the wave waiting)
+    for (uint32_t j0 = 0; j0 < total; j0 += kWave * kPeelUnroll) {
+      uint32_t xa[kPeelUnroll], xb[kPeelUnroll];  // partner edges (u, w) and (v, w) to decrement
+#pragma unroll
+      for (uint32_t k = 0; k < kPeelUnroll; ++k) {
+        xa[k] = kNone;
+        xb[k] = kNone;
+        const uint32_t j = j0 + k * kWave + uint32_t(lane);
+        if (j < total) {
+          uint32_t lo = 0, hi = kWave - 1;  // owner: first lane whose inclusive count exceeds j
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_inc[wbase + mid] > j)
+              hi = mid;
+            else
+              lo = mid + 1;
+          }
+          const unsigned long long tw = s_t[wbase + lo];
+          const uint32_t b = select_bit(tw, j - (s_inc[wbase + lo] - uint32_t(__popcll(tw))));
+          const uint32_t w = ((ch * cw + lo) << 6) + b;
+          const bool ina = (s_fa[wbase + lo] >> b) & 1ull, inb = (s_fb[wbase + lo] >> b) & 1ull;
+          if (!ina && !inb) {
+            xa[k] = eid_of(u, w);
+            xb[k] = eid_of(v, w);
+          } else if (ina) {  // (u, w) in the frontier, third edge (v, w)
+            if (w < v) xb[k] = eid_of(v, w);
+          } else {  // (v, w) in the frontier, third edge (u, w)
+            if (w < u) xa[k] = eid_of(u, w);
+          }
         }
       }
-      const bool pa = dec_crosses(xa, thr, sup);
-      const bool pb = dec_crosses(xb, thr, sup);
-      wave_append(pa, xa, qn, qn_count);
-      wave_append(pb, xb, qn, qn_count);
+      int32_t oa[kPeelUnroll], ob[kPeelUnroll];  // supports before the decrements
+#pragma unroll
+      for (uint32_t k = 0; k < kPeelUnroll; ++k) {
+        oa[k] = xa[k] != kNone ? atomicSub(&sup[xa[k]], 1) : 0;
+        ob[k] = xb[k] != kNone ? atomicSub(&sup[xb[k]], 1) : 0;
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kPeelUnroll; ++k) {  // a partner that falls below thr now joins the next frontier
+        wave_append(xa[k] != kNone && oa[k] == thr, xa[k], qn, qn_count);
+        wave_append(xb[k] != kNone && ob[k] == thr, xb[k], qn, qn_count);
+      }
     }
     wave_sync();
   }
