@@ -32,7 +32,7 @@ namespace {
 // the edges, for a level's minimum support).
 constexpr int kStampGrid = 256;
 constexpr int kPeelGrid = 1024;
-constexpr uint32_t kPeelUnroll = 4;  // triangles per lane per step of the dense peel
+constexpr uint32_t kPeelUnroll = 8;  // triangles per lane per step of the dense peel
 constexpr int kClearGrid = 512;
 
 // What the next round does with a frontier of c edges when `alive` edges are
