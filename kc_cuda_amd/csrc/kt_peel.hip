@@ -32,7 +32,8 @@ namespace {
 // the edges, for a level's minimum support).
 constexpr int kStampGrid = 256;
 constexpr int kPeelGrid = 1024;
-constexpr uint32_t kPeelUnroll = 8;  // triangles per lane per step of the dense peel
+constexpr uint32_t kPeelUnroll = 8;      // triangles per lane per step of the dense peel
+constexpr uint32_t kSparseStage = 4096;  // LDS words per workgroup for the sparse peel's staged rows
 constexpr int kClearGrid = 512;
 
 // What the next round does with a frontier of c edges when `alive` edges are
@@ -435,6 +436,11 @@ __global__ __launch_bounds__(kBlock) void k_sparse_peel_round(
   uint32_t* qn_count = &ctl->qcount[cur ^ 1u];
   const uint32_t groups = gridDim.x * (blockDim.x / G);
   const uint32_t lane = threadIdx.x % G;
+  // the longer row of each group's edge is staged in LDS when it fits, so
+  // its binary searches cost LDS reads instead of dependent HBM round trips
+  constexpr uint32_t kCap = kSparseStage / (kBlock / G);
+  __shared__ uint32_t s_rows[kSparseStage];
+  uint32_t* sb = s_rows + (threadIdx.x / G) * kCap;
   // every lane of a wave runs the same number of steps so queue appends can
   // be wave-aggregated
   const uint32_t steps = (cnt + groups - 1) / groups;
@@ -457,13 +463,19 @@ __global__ __launch_bounds__(kBlock) void k_sparse_peel_round(
     uint32_t len = ae - ab;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) len = max(len, uint32_t(__shfl_xor(len, o)));
+    const uint32_t nb = be - bb;
+    const bool staged = nb <= kCap;
+    wave_sync();  // the previous step's searches are done with the stage
+    if (staged && ab < ae)
+      for (uint32_t j = lane; j < nb; j += G) sb[j] = adj_u[bb + j];
+    wave_sync();
     for (uint32_t i0 = 0; i0 < len; i0 += G) {
       const uint32_t i = ab + i0 + lane;
       uint32_t xa = kNone, xb = kNone;
       if (i < ae) {
         const uint32_t x = adj_u[i];
-        const uint32_t p = lower_bound_u32(adj_u, bb, be, x);
-        if (p < be && adj_u[p] == x) {
+        const uint32_t p = staged ? bb + lower_bound_u32(sb, 0u, nb, x) : lower_bound_u32(adj_u, bb, be, x);
+        if (p < be && (staged ? sb[p - bb] : adj_u[p]) == x) {
           const uint32_t ea = eid_u[i], eb = eid_u[p];
           const uint32_t ma = state[ea], mb = state[eb];
           if (ma >= r && mb >= r) {  // neither side deleted in an earlier round
