@@ -1028,6 +1028,37 @@ __device__ __forceinline__ uint32_t alive_bit(const uint32_t* __restrict__ bits,
   return coarse_hit(coarse, w) ? (bits[w >> 5] >> (w & 31)) & 1u : 0u;
 }
 
+// kCutUnroll row slots per lane at once: each level of the lookup (row
+// entry, coarse word, fine word, prefix) is loaded for all of them before
+// any is used, so their round trips overlap. ok[u] marks the slots in range;
+// nid[u] = the neighbour's new id, or kNone when it is not alive (with
+// pref == nullptr: 1 / kNone for alive / not).
+constexpr uint32_t kCutUnroll = 4;
+__device__ __forceinline__ void alive_ranks(const uint32_t* __restrict__ adj, uint64_t j0, uint64_t step,
+                                            uint64_t e, const uint32_t* __restrict__ bits,
+                                            const uint32_t* __restrict__ pref, const uint32_t* __restrict__ coarse,
+                                            uint32_t* nid) {
+  uint32_t w[kCutUnroll], c[kCutUnroll], b[kCutUnroll];
+#pragma unroll
+  for (uint32_t u = 0; u < kCutUnroll; ++u) {
+    const uint64_t j = j0 + u * step;
+    w[u] = j < e ? adj[j] : kNone;
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < kCutUnroll; ++u) c[u] = w[u] != kNone ? coarse[w[u] >> 10] : 0u;
+#pragma unroll
+  for (uint32_t u = 0; u < kCutUnroll; ++u) {
+    b[u] = 0;
+    if ((c[u] >> ((w[u] >> 5) & 31)) & 1u) b[u] = bits[w[u] >> 5];
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < kCutUnroll; ++u) {
+    nid[u] = kNone;
+    const uint32_t m = 1u << (w[u] & 31);
+    if (w[u] != kNone && (b[u] & m)) nid[u] = pref ? pref[w[u] >> 5] + uint32_t(__popc(b[u] & (m - 1u))) : 1u;
+  }
+}
+
 // Degrees of the cut counted from the rows themselves: sub_deg[i] = the
 // neighbours of list[i] whose bit is set in the alive bitmap (n / 8 bytes,
 // so the lookups stay in L2 and the MALL). One wave per listed vertex; rows
@@ -1052,7 +1083,12 @@ __global__ __launch_bounds__(kBlock) void k_list_alive_deg(const uint64_t* __res
       continue;
     }
     uint32_t c = 0;
-    for (uint64_t j = b + lane; j < e; j += kWave) c += alive_bit(abits, acoarse, adj[j]);
+    for (uint64_t j0 = b + lane; j0 < e; j0 += kWave * kCutUnroll) {
+      uint32_t nid[kCutUnroll];
+      alive_ranks(adj, j0, kWave, e, abits, nullptr, acoarse, nid);
+#pragma unroll
+      for (uint32_t u = 0; u < kCutUnroll; ++u) c += nid[u] != kNone;
+    }
     c = wave_sum(c);
     if (lane == 0) sub_deg[i] = c;
   }
@@ -1077,7 +1113,12 @@ __global__ __launch_bounds__(kBlock) void k_list_alive_deg_heavy(const uint64_t*
     const uint64_t b = off[v] + uint64_t(t - slices[h]) * kSlice;
     const uint64_t e = min(off[v + 1], b + kSlice);
     uint32_t c = 0;
-    for (uint64_t j = b + threadIdx.x; j < e; j += blockDim.x) c += alive_bit(abits, acoarse, adj[j]);
+    for (uint64_t j0 = b + threadIdx.x; j0 < e; j0 += uint64_t(blockDim.x) * kCutUnroll) {
+      uint32_t nid[kCutUnroll];
+      alive_ranks(adj, j0, blockDim.x, e, abits, nullptr, acoarse, nid);
+#pragma unroll
+      for (uint32_t u = 0; u < kCutUnroll; ++u) c += nid[u] != kNone;
+    }
     c = wave_sum(c);
     if (lane64() == 0 && c) atomicAdd(&sub_deg[i], c);
   }
@@ -1107,17 +1148,20 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __res
     const uint32_t v = list[i];
     const uint64_t b = off[v] + uint64_t(t - slices[h]) * kSlice;
     const uint64_t e = min(off[v + 1], b + kSlice);
-    for (uint64_t j0 = b + (threadIdx.x & ~63u); j0 < e; j0 += blockDim.x) {
-      const uint64_t j = j0 + lane;
-      const uint32_t nid = j < e ? alive_rank(abits, apref, acoarse, adj[j]) : kNone;
-      const bool p = nid != kNone;
-      const uint64_t mask = __ballot(p);
-      if (!mask) continue;
-      const int leader = __ffsll((unsigned long long)mask) - 1;
-      unsigned long long at = 0;
-      if (lane == leader) at = atomicAdd(&cursor[h], (unsigned long long)__popcll(mask));
-      at = __shfl(at, leader);
-      if (p) sub_adj[sub_off[i] + at + uint64_t(__popcll(mask & ((1ull << lane) - 1ull)))] = nid;
+    for (uint64_t j0 = b + (threadIdx.x & ~63u); j0 < e; j0 += uint64_t(blockDim.x) * kCutUnroll) {
+      uint32_t nid[kCutUnroll];
+      alive_ranks(adj, j0 + lane, blockDim.x, e, abits, apref, acoarse, nid);
+#pragma unroll
+      for (uint32_t u = 0; u < kCutUnroll; ++u) {
+        const bool p = nid[u] != kNone;
+        const uint64_t mask = __ballot(p);
+        if (!mask) continue;
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        unsigned long long at = 0;
+        if (lane == leader) at = atomicAdd(&cursor[h], (unsigned long long)__popcll(mask));
+        at = __shfl(at, leader);
+        if (p) sub_adj[sub_off[i] + at + uint64_t(__popcll(mask & ((1ull << lane) - 1ull)))] = nid[u];
+      }
     }
   }
 }
@@ -1141,13 +1185,16 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict_
       if (lane == 0) push_heavy(i, heavy, heavy_count);
       continue;
     }
-    for (uint64_t j0 = off[v]; j0 < e; j0 += kWave) {
-      const uint64_t j = j0 + lane;
-      const uint32_t nid = j < e ? alive_rank(abits, apref, acoarse, adj[j]) : kNone;
-      const bool p = nid != kNone;
-      const uint64_t mask = __ballot(p);
-      if (p) sub_adj[pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = nid;
-      pos += uint32_t(__popcll(mask));
+    for (uint64_t j0 = off[v]; j0 < e; j0 += kWave * kCutUnroll) {
+      uint32_t nid[kCutUnroll];  // slots j0 + u * 64 + lane, in row order
+      alive_ranks(adj, j0 + lane, kWave, e, abits, apref, acoarse, nid);
+#pragma unroll
+      for (uint32_t u = 0; u < kCutUnroll; ++u) {
+        const bool p = nid[u] != kNone;
+        const uint64_t mask = __ballot(p);
+        if (p) sub_adj[pos + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = nid[u];
+        pos += uint32_t(__popcll(mask));
+      }
     }
   }
 }
