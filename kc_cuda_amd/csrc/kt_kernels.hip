@@ -11,6 +11,10 @@
 // queues replace full-array mask scans, hub rows live in LDS as a bitmap plus
 // a sorted list, and same-row credits are accumulated in LDS.
 #include <hip/hip_ext.h>
+
+#include <algorithm>
+#include <cstdlib>
+
 #include "kt_device.h"
 #include "kt_kernels.h"
 
@@ -777,14 +781,13 @@ __global__ void k_core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k) {
 // the decrement that takes it from k to k - 1 queues it (an LDS atomic, so
 // no vertex is queued twice). When a threshold removes every alive vertex,
 // the vertices it removed are the max core: they are re-opened (lvl = kNone)
-// and their degrees recounted among themselves. A removed vertex's LDS
+// with the degrees they had when that threshold started. A removed vertex's LDS
 // degree carries bit 31; afterwards lvl[v] = 1 + the core number of every
 // vertex removed here, so every threshold from k0 up is an exact cut.
 constexpr uint32_t kDecompBlock = 1024;
 constexpr uint32_t kDecompMaxN = 16000;   // LDS degrees: 64,000 B
 constexpr uint32_t kDecompUnroll = 8;     // row slots a lane loads before its atomics
 constexpr uint32_t kGone = 0x80000000u;
-constexpr uint32_t kCoreMark = 0x40000000u;
 __global__ __launch_bounds__(kDecompBlock) void k_core_decompose(const uint64_t* __restrict__ off,
                                                                   const uint32_t* __restrict__ adj, uint32_t n,
                                                                   uint32_t k0, uint32_t* lvl, uint32_t* deg,
@@ -812,10 +815,13 @@ __global__ __launch_bounds__(kDecompBlock) void k_core_decompose(const uint64_t*
   uint32_t start = 0;               // queue position where threshold k's removals begin
   bool done = s_alive == 0;
   while (!done) {
-    // the threshold's first vertices
+    // the threshold's first vertices; the alive degrees are kept in deg, so
+    // the threshold that empties the level leaves the max core's degrees there
     for (uint32_t v = tid; v < n; v += kDecompBlock) {
       const uint32_t d = s_deg[v];
-      if (d < k) {  // alive (bit 31 clear) with degree < k
+      if (d & kGone) continue;
+      deg[v] = d;
+      if (d < k) {
         s_deg[v] = d | kGone;
         lvl[v] = k;
         q[atomicAdd(&s_tail, 1u)] = v;
@@ -870,30 +876,10 @@ __global__ __launch_bounds__(kDecompBlock) void k_core_decompose(const uint64_t*
       __syncthreads();
     }
   }
-  // the last threshold's removals (queue [start, s_tail)) are the max core
+  // the last threshold's removals (queue [start, s_tail)) are the max core;
+  // deg holds their degrees at that threshold's start, i.e. within the core
   const uint32_t tail = s_tail;
-  for (uint32_t i = start + tid; i < tail; i += kDecompBlock) s_deg[q[i]] = kCoreMark;
-  __syncthreads();
-  for (uint32_t i = start + wid; i < tail; i += kWaves) {
-    const uint32_t v = q[i];
-    const uint64_t b = off[v], e = off[v + 1];
-    uint32_t c = 0;
-    for (uint64_t j0 = b; j0 < e; j0 += kWave * kDecompUnroll) {
-      uint32_t w[kDecompUnroll];
-#pragma unroll
-      for (uint32_t u = 0; u < kDecompUnroll; ++u) {
-        const uint64_t j = j0 + u * kWave + lane;
-        w[u] = j < e ? adj[j] : kNone;
-      }
-#pragma unroll
-      for (uint32_t u = 0; u < kDecompUnroll; ++u) c += w[u] != kNone && s_deg[w[u]] == kCoreMark;
-    }
-    c = wave_sum(c);
-    if (lane == 0) {
-      deg[v] = c;
-      lvl[v] = kNone;
-    }
-  }
+  for (uint32_t i = start + tid; i < tail; i += kDecompBlock) lvl[q[i]] = kNone;
   if (tid == 0) {
     out[0] = tail > start ? k - 1 : 0;  // the max core's k (0: nothing was alive)
     out[1] = tail - start;              // its vertices
@@ -1672,10 +1658,14 @@ bool dense_fits(uint32_t nc, uint32_t mc) {
   // a bitmap row (ws words of 8 bytes) no larger than two average sorted rows
   return uint64_t(dense_words(nc)) * nc <= 2ull * mc;
 }
-static int dense_lanes(uint32_t ws) {
-  if (ws >= 48) return 64;
-  if (ws >= 24) return 32;
-  if (ws >= 8) return 16;
+static int dense_lanes(uint32_t ws) {  // tuning only: KT_DENSE_COUNT_WORDS words per lane
+  static const uint32_t per = [] {
+    const char* v = std::getenv("KT_DENSE_COUNT_WORDS");
+    return v ? uint32_t(std::max(1, std::atoi(v))) : 8u;
+  }();
+  if (ws >= 64 * per) return 64;
+  if (ws >= 32 * per) return 32;
+  if (ws >= 16 * per / 2) return 16;
   return 8;
 }
 void dense_rows(const uint32_t* off_u, const uint32_t* adj_u, uint32_t nc, uint32_t ws, uint64_t* bits0,
