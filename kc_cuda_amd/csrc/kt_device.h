@@ -136,19 +136,25 @@ __host__ __device__ __forceinline__ uint32_t select_bit(unsigned long long x, ui
 
 // Last workgroup of a launch to arrive here (every workgroup must call it
 // exactly once, with all its threads). The counter is reset by the winner.
+// No fence: an agent-scope fence writes back and invalidates the whole L2 of
+// the workgroup's XCD on this architecture, which cost every round kernel
+// its cached rows. What a last workgroup needs from the others travels
+// through device atomics, which are performed past the L2: each workgroup
+// waits for its own (returning) atomics before taking its ticket, and the
+// last one reads the results with coherent loads (ld_agent). Plain stores
+// are for later launches, which the launch boundary orders.
 __device__ __forceinline__ bool last_block(uint32_t* ticket) {
   __shared__ bool s_last;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    s_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-  }
+  if (threadIdx.x == 0) s_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
   __syncthreads();
-  if (s_last) {
-    __threadfence();
-    if (threadIdx.x == 0) *ticket = 0;
-  }
+  if (s_last && threadIdx.x == 0) atomicExch(ticket, 0u);
   return s_last;
+}
+
+// Load that reads the device-coherent value (past this XCD's L2).
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace

@@ -102,12 +102,15 @@ __device__ __forceinline__ void level_end(Ctl* ctl, const uint32_t* __restrict__
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int k = 1; k < kBlock / kWave; ++k) best = min(best, s_min[k]);
-    if (best != kNone) atomicMin(&ctl->minsup, best);
+    if (best != kNone) {
+      const uint32_t prev = atomicMin(&ctl->minsup, best);
+      asm volatile("" ::"v"(prev));  // performed before this workgroup's ticket
+    }
   }
   if (last_block(&ctl->ticket) && threadIdx.x == 0) {
     switch_queue();
-    const uint32_t ms = ctl->minsup;
-    ctl->minsup = kNone;
+    const uint32_t ms = ld_agent(&ctl->minsup);
+    atomicExch(&ctl->minsup, kNone);
     const unsigned long long alive = ctl->alive;
     if (ms == kNone || alive == 0) {  // nothing live: the search is over
       if ((ms == kNone) != (alive == 0)) ctl->err |= kErrPeelCount;
