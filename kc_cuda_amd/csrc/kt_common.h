@@ -112,6 +112,7 @@ struct RoundRule {
   uint32_t pad;
   double tri_cost;    // dense: cost of one triangle visit in word ANDs
   double drop_ratio;  // sparse: drop when frontier * ratio >= alive
+  double drop_fixed;  // dense: fixed cost of a drop (host round trip, restart) in word ANDs
 };
 
 // How a truss search ends: early exit on a first level of early_ans edges at

@@ -44,7 +44,7 @@ __device__ __forceinline__ uint32_t decide_round(uint32_t c, unsigned long long 
   if (allow_compact && 2ull * alive <= rule.mc) return kRoundCompact;
   if (c == alive) return kRoundEmpty;
   const bool drop = rule.dense ? double(c) * (double(rule.ws) + rule.tri_cost * double(thr)) >=
-                                     double(alive) * double(rule.ws)
+                                     double(alive) * double(rule.ws) + rule.drop_fixed
                                : double(c) * rule.drop_ratio >= double(alive);
   return drop ? kRoundDrop : kRoundPeel;
 }

@@ -311,7 +311,8 @@ class Solver {
   std::vector<uint64_t> deg_at_least_;  // input degree histogram, suffix sums per bin (empty: none)
   DevBuf bk_state_, bk_sup_, bk_bits_, bk_del_;  // threshold-probe backups
   uint64_t eidm_budget_ = uint64_t(4) << 30;  // bytes for the edge-id matrix
-  double dense_tri_cost_ = 4.0, sparse_drop_ratio_ = 3.0, default_drop_ratio_ = 3.0;
+  double dense_tri_cost_ = 10.0, sparse_drop_ratio_ = 3.0, default_drop_ratio_ = 3.0;
+  double dense_drop_fixed_ = 32e6;  // about what a drop's host round trip and restart cost
   bool trace_ = false;  // KT_TRACE=1: per-round frontier sizes on stderr
   // trace only: milliseconds since the previous call, after the stream drains
   double trace_ms();

@@ -118,6 +118,7 @@ void Solver::init_common() {
   KT_HIP(hipStreamSynchronize(stream_));
   // cost-model knobs of the frontier-versus-recount choice (tuning only)
   if (const char* v = std::getenv("KT_DENSE_TRI_COST")) dense_tri_cost_ = std::atof(v);
+  if (const char* v = std::getenv("KT_DENSE_DROP_FIXED")) dense_drop_fixed_ = std::atof(v);
   if (const char* v = std::getenv("KT_SPARSE_DROP_RATIO")) sparse_drop_ratio_ = std::atof(v);
   if (const char* v = std::getenv("KT_TRACE")) trace_ = v[0] == '1';
   if (const char* v = std::getenv("KT_EIDM_BUDGET_MB")) eidm_budget_ = uint64_t(std::atoll(v)) << 20;

@@ -602,6 +602,7 @@ RoundRule Solver::round_rule(const CoreView& cv) const {
   rule.ws = cv.ws;
   rule.mc = cv.mc;
   rule.tri_cost = dense_tri_cost_;
+  rule.drop_fixed = dense_drop_fixed_;
   rule.drop_ratio = sparse_drop_ratio_;
   return rule;
 }
