@@ -131,7 +131,10 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
  * (0 = automatic; 8, 16, 32 or 64) and kernel-variant flags:
  * KT_FORCE_EDGE_COUNT selects the per-edge support kernel instead of the LDS
  * bitmap one, KT_FORCE_ATOMIC_BUILD the atomic CSR build instead of the
- * bucketed one, KT_FORCE_SPARSE / KT_FORCE_DENSE the sorted-row or the
+ * bucketed one, KT_FORCE_UNSORTED_BUILD the bucketed build even for inputs
+ * sorted by row (default: lines sorted by row are read as runs, and pairs
+ * sorted by their first id have only their second entries bucketed),
+ * KT_FORCE_SPARSE / KT_FORCE_DENSE the sorted-row or the
  * bit-matrix core representation (default: by density), KT_FORCE_LEVEL_WALK
  * the level-by-level first pass on bit-matrix cores (default: 16 levels are
  * walked, then thresholds are probed), KT_FORCE_PROBES threshold probes right
@@ -151,6 +154,7 @@ int kt_parse_tsv(const char* path, uint32_t* xs, uint32_t* ys, uint64_t cap, uin
 #define KT_FORCE_PROBES 32
 #define KT_FORCE_VERTEX_ROUNDS 64
 #define KT_FORCE_VERTEX_CASCADE 128
+#define KT_FORCE_UNSORTED_BUILD 256
 int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
 
 /* Batched solve of `count` independent graphs (no counterpart in the

@@ -252,17 +252,19 @@ class Context:
     # -- solves -------------------------------------------------------------
     def tune(self, count_lanes: int = 0, peel_lanes: int = 0, force_edge_count: bool = False,
              force_atomic_build: bool = False, core: str = "auto", pass1: str = "auto",
-             vertex: str = "auto"):
+             vertex: str = "auto", unsorted_build: bool = False):
         """core: "auto", "sparse" (sorted rows) or "dense" (bit-matrix rows);
         pass1 on bit-matrix cores: "auto" (walk 16 levels, then probe
         thresholds), "walk" (every level) or "probe" (probe after level 1);
         vertex: the vertex peel as one "cascade" launch per threshold, as
         synchronous "rounds", or "auto" (the cascade on levels of up to
-        2^24 row slots)."""
+        2^24 row slots); unsorted_build: the bucketed CSR build even for
+        inputs sorted by row."""
         flags = (1 if force_edge_count else 0) | (2 if force_atomic_build else 0)
         flags |= {"auto": 0, "sparse": 4, "dense": 8}[core]
         flags |= {"auto": 0, "walk": 16, "probe": 32}[pass1]
         flags |= {"auto": 0, "rounds": 64, "cascade": 128}[vertex]
+        flags |= 256 if unsorted_build else 0
         _check(self._lib.kt_set_tuning(self._h, count_lanes, peel_lanes, flags))
 
     def solve_pairs(self, pairs: np.ndarray, n: int) -> KtResult:

@@ -266,6 +266,7 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags) {
   ctx->solver->set_lanes(count_lanes, peel_lanes);
   ctx->solver->set_force_edge_count((flags & KT_FORCE_EDGE_COUNT) != 0);
   ctx->solver->set_force_atomic_build((flags & KT_FORCE_ATOMIC_BUILD) != 0);
+  ctx->solver->set_sorted_build((flags & KT_FORCE_UNSORTED_BUILD) == 0);
   ctx->solver->set_core_mode((flags & KT_FORCE_SPARSE) ? 1 : (flags & KT_FORCE_DENSE) ? 2 : 0);
   ctx->solver->set_gallop((flags & KT_FORCE_LEVEL_WALK) == 0, (flags & KT_FORCE_PROBES) ? 1u : 16u);
   ctx->solver->set_vertex_mode((flags & KT_FORCE_VERTEX_ROUNDS) ? 1 : (flags & KT_FORCE_VERTEX_CASCADE) ? 2 : 0);

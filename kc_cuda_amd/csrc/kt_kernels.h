@@ -22,7 +22,8 @@ void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStr
 
 // Partitioned build: entries bucketed by row >> S (B buckets, T tiles of
 // `tile` items), then rows built per bucket in LDS. mode 0: a = interleaved
-// pairs (both directions); mode 1: a = xs (columns), b = ys (rows).
+// pairs (both directions); mode 1: a = xs (columns), b = ys (rows); mode 2:
+// pairs, second entries (row v, column u) only.
 // hist_t has T*B entries (tile-major, one row per tile); its transpose is
 // scanned bucket-major into hoff (B*T+1), whose tile-major transpose hoff_t
 // gives each tile its write cursors.
@@ -35,8 +36,24 @@ void transpose_u64(const uint64_t* in, uint32_t rows, uint32_t cols, uint64_t* o
 void csr_part_scatter(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile,
                       uint32_t n, uint32_t S, uint32_t B, uint32_t T, const uint64_t* hoff_t,
                       uint32_t* part, hipStream_t s);
-void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, uint32_t B, uint32_t T, uint32_t S,
-                     uint32_t n, uint64_t* off, uint32_t* deg, uint32_t* adj, hipStream_t s);
+// fwd_pairs / f0 (row-sorted pairs, else null): the pairs' first entries,
+// read in place (f0 from csr_fwd_start); part then holds only their second
+// entries (mode 2 of csr_part_hist / csr_part_scatter).
+void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, const uint32_t* fwd_pairs, const uint64_t* f0,
+                     uint32_t B, uint32_t T, uint32_t S, uint32_t n, uint64_t* off, uint32_t* deg, uint32_t* adj,
+                     hipStream_t s);
+// Row-sorted inputs. csr_sorted_check: *unsorted |= 1 unless the rows are
+// non-decreasing (mode 0: pairs by first id; mode 1: lines by ys); for lines
+// also hpos[r] / tpos[r] = first / one past the last line of row r (fill
+// both with 0 first) and *err |= 1 for a row >= n. csr_deg_from_runs:
+// deg[0..n] from them (deg[n] = 0). csr_copy_cols: adj = xs (*err |= 1 for a
+// column >= n). csr_fwd_start: f0[b] = first pair with first id >= b << S,
+// f0[B] = items.
+void csr_sorted_check(int mode, const uint32_t* keys, uint64_t items, uint32_t n, uint64_t* hpos, uint64_t* tpos,
+                      uint32_t* unsorted, uint32_t* err, hipStream_t s);
+void csr_deg_from_runs(const uint64_t* hpos, const uint64_t* tpos, uint32_t n, uint32_t* deg, hipStream_t s);
+void csr_copy_cols(const uint32_t* xs, uint64_t L, uint32_t n, uint32_t* adj, uint32_t* err, hipStream_t s);
+void csr_fwd_start(const uint32_t* pairs, uint64_t items, uint32_t S, uint32_t B, uint64_t* f0, hipStream_t s);
 
 // (b) vertex peeling (k-core). lvl[v] == kNone: alive; otherwise the
 //     threshold at which v was removed. deg holds alive-neighbour counts.

@@ -147,7 +147,10 @@ __device__ __forceinline__ LaneRun lane_run(uint32_t key, bool active) {
   return r;
 }
 
-template <int Mode>
+// kBwd (pairs only): count and scatter only each pair's second entry (row v,
+// column u); the first entries of row-sorted pairs are placed in order by
+// k_bucket_rows_sorted instead.
+template <int Mode, bool kBwd>
 __global__ __launch_bounds__(kBlock) void k_part_hist(EntrySrc<Mode> src, uint64_t tile, uint32_t n,
                                                       uint32_t S, uint32_t B, uint32_t* hist_t,
                                                       uint32_t* err) {
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(EntrySrc<Mode> src, uint64
     uint32_t rows[2] = {0, 0}, cols[2] = {0, 0};
     const bool in = i < hi;
     if (in) src.get(i, rows, cols);
-    for (int k = 0; k < src.per_item(); ++k) {
+    for (int k = kBwd ? 1 : 0; k < src.per_item(); ++k) {
       bool ok = in;
       if (in && (rows[k] >= n || cols[k] >= n)) {
         atomicOr(err, 1u);
@@ -196,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const T* __restrict__ in, 
   }
 }
 
-template <int Mode>
+template <int Mode, bool kBwd>
 __global__ __launch_bounds__(kBlock) void k_part_scatter(EntrySrc<Mode> src, uint64_t tile, uint32_t n,
                                                          uint32_t S, uint32_t B,
                                                          const uint64_t* __restrict__ hoff_t, uint2* part) {
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(EntrySrc<Mode> src, uin
     uint32_t rows[2] = {0, 0}, cols[2] = {0, 0};
     const bool in = i < hi;
     if (in) src.get(i, rows, cols);
-    for (int k = 0; k < src.per_item(); ++k) {
+    for (int k = kBwd ? 1 : 0; k < src.per_item(); ++k) {
       const bool ok = in && rows[k] < n && cols[k] < n;
       const uint32_t b = rows[k] >> S;
       const LaneRun r = lane_run(b, ok);
@@ -224,10 +227,17 @@ __global__ __launch_bounds__(kBlock) void k_part_scatter(EntrySrc<Mode> src, uin
   }
 }
 
-// One workgroup per bucket: row counts in LDS, block scan, placement.
+// One workgroup per bucket: row counts in LDS, block scan, placement. The
+// bucket's entries are part[hoff(b) .. hoff(b + 1)), plus, for row-sorted
+// pairs (fwd non-null), the first entries (row u, column v) of the pairs in
+// fwd[f0[b] .. f0[b + 1]), which lie there in row order and are read in
+// place instead of being scattered; the bucket's rows then start at
+// hoff(b) + f0[b]. A row's slots are dealt from one LDS cursor, in any order.
 constexpr int kBucketBlock = 1024;
 __global__ __launch_bounds__(kBucketBlock) void k_bucket_rows(const uint2* __restrict__ part,
                                                               const uint64_t* __restrict__ hoff,
+                                                              const uint2* __restrict__ fwd,
+                                                              const uint64_t* __restrict__ f0,
                                                               uint32_t T, uint32_t S, uint32_t n,
                                                               uint64_t* off, uint32_t* deg, uint32_t* adj) {
   extern __shared__ uint32_t cnt[];
@@ -237,15 +247,22 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket_rows(const uint2* __res
   const uint32_t r0 = b << S;
   const uint32_t nrows = min(R, n - r0);
   const uint64_t beg = hoff[uint64_t(b) * T], end = hoff[uint64_t(b + 1) * T];
+  const uint64_t fb = fwd ? f0[b] : 0, fe = fwd ? f0[b + 1] : 0;
+  const uint64_t base = beg + fb;  // the bucket's first slot in adj
+  const int lane = lane64();
   for (uint32_t i = threadIdx.x; i < R; i += blockDim.x) cnt[i] = 0;
   __syncthreads();
-  for (uint64_t i0 = beg; i0 < end; i0 += blockDim.x) {  // block-uniform trip count
-    const uint64_t i = i0 + threadIdx.x;
-    const bool in = i < end;
-    const uint32_t r = in ? part[i].x - r0 : 0u;
-    const LaneRun run = lane_run(r, in);
-    if (run.is_head) atomicAdd(&cnt[r], run.len);
-  }
+  auto count = [&](const uint2* __restrict__ src, uint64_t lo, uint64_t hi) {
+    for (uint64_t i0 = lo; i0 < hi; i0 += blockDim.x) {  // block-uniform trip count
+      const uint64_t i = i0 + threadIdx.x;
+      const bool in = i < hi;
+      const uint32_t r = in ? src[i].x - r0 : 0u;
+      const LaneRun run = lane_run(r, in);
+      if (run.is_head) atomicAdd(&cnt[r], run.len);
+    }
+  };
+  count(part, beg, end);
+  if (fwd) count(fwd, fb, fe);
   __syncthreads();
   // exclusive scan of cnt[0..R): each thread owns R / blockDim consecutive rows
   const uint32_t per = R / blockDim.x;
@@ -253,7 +270,6 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket_rows(const uint2* __res
   uint32_t local = 0;
   for (uint32_t k = 0; k < per; ++k) local += cnt[base_i + k];
   // wave-inclusive scan of `local`
-  const int lane = lane64();
   uint32_t inc = local;
 #pragma unroll
   for (int o = 1; o < kWave; o <<= 1) {
@@ -278,23 +294,102 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket_rows(const uint2* __res
     const uint32_t r = base_i + k;
     const uint32_t c = cnt[r];
     if (r < nrows) {
-      off[r0 + r] = beg + run;
+      off[r0 + r] = base + run;
       deg[r0 + r] = c;
     }
     cnt[r] = run;  // becomes the placement cursor
     run += c;
   }
   __syncthreads();
-  for (uint64_t i0 = beg; i0 < end; i0 += blockDim.x) {  // block-uniform trip count
-    const uint64_t i = i0 + threadIdx.x;
-    const bool in = i < end;
-    const uint2 e = in ? part[i] : make_uint2(r0, 0u);
-    const LaneRun run = lane_run(e.x - r0, in);
-    uint32_t p = 0;
-    if (run.is_head) p = atomicAdd(&cnt[e.x - r0], run.len);
-    p = __shfl(p, in ? run.head : lane);
-    if (in) adj[beg + p + uint32_t(lane - run.head)] = e.y;
+  auto place = [&](const uint2* __restrict__ src, uint64_t lo, uint64_t hi) {
+    for (uint64_t i0 = lo; i0 < hi; i0 += blockDim.x) {  // block-uniform trip count
+      const uint64_t i = i0 + threadIdx.x;
+      const bool in = i < hi;
+      const uint2 e = in ? src[i] : make_uint2(r0, 0u);
+      const LaneRun run = lane_run(e.x - r0, in);
+      uint32_t p = 0;
+      if (run.is_head) p = atomicAdd(&cnt[e.x - r0], run.len);
+      p = __shfl(p, in ? run.head : lane);
+      if (in) adj[base + p + uint32_t(lane - run.head)] = e.y;
+    }
+  };
+  place(part, beg, end);
+  if (fwd) place(fwd, fb, fe);
+}
+
+// ------------------------------------------- row-sorted inputs (a) ---
+// The reference's edge-list contract lists lines sorted by row
+// (src/kc_gpu.cpp:243-312 relies on it), and canonical pairs usually come
+// sorted by their first id. Such inputs skip the scatter of the lines (their
+// rows are runs already) or of the pairs' first entries.
+
+// *unsorted becomes non-zero when keys[i * Stride] > keys[(i + 1) * Stride]
+// for some i. With hpos / tpos (lines), each run of equal rows r < n also
+// records hpos[r] = its first index and tpos[r] = its last index + 1 (valid
+// when the keys are sorted; rows without lines keep what the caller
+// filled); a row >= n sets *err.
+template <int Stride>
+__global__ __launch_bounds__(kBlock) void k_sorted_runs(const uint32_t* __restrict__ keys, uint64_t items,
+                                                        uint32_t n, uint64_t* hpos, uint64_t* tpos,
+                                                        uint32_t* unsorted, uint32_t* err) {
+  bool bad = false, big = false;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < items; i += stride) {
+    const uint32_t k = keys[i * Stride];
+    const bool has_next = i + 1 < items;
+    const uint32_t nx = has_next ? keys[(i + 1) * Stride] : 0u;
+    bad |= has_next && nx < k;
+    if (hpos) {
+      if (k >= n) {
+        big = true;
+        continue;
+      }
+      if (i == 0 || keys[(i - 1) * Stride] != k) hpos[k] = i;
+      if (!has_next || nx != k) tpos[k] = i + 1;
+    }
   }
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(unsorted, 1u);
+  if (hpos && __syncthreads_or(big) && threadIdx.x == 0) atomicOr(err, 1u);
+}
+
+// deg[r] = tpos[r] - hpos[r] for r < n, deg[n] = 0.
+__global__ void k_deg_from_runs(const uint64_t* __restrict__ hpos, const uint64_t* __restrict__ tpos, uint32_t n,
+                                uint32_t* deg) {
+  for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r <= n; r += uint64_t(gridDim.x) * blockDim.x)
+    deg[r] = r < n ? uint32_t(tpos[r] - hpos[r]) : 0u;
+}
+
+// adj[i] = xs[i]; a column >= n sets *err.
+__global__ __launch_bounds__(kBlock) void k_copy_cols(const uint32_t* __restrict__ xs, uint64_t L, uint32_t n,
+                                                      uint32_t* __restrict__ adj, uint32_t* err) {
+  bool big = false;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < L; i += stride) {
+    const uint32_t x = xs[i];
+    big |= x >= n;
+    adj[i] = x;
+  }
+  if (__syncthreads_or(big) && threadIdx.x == 0) atomicOr(err, 1u);
+}
+
+// f0[b] = index of the first pair whose first id is >= b << S (pairs sorted
+// by their first id), for b < B; f0[B] = items.
+__global__ void k_fwd_start(const uint2* __restrict__ pairs, uint64_t items, uint32_t S, uint32_t B,
+                            uint64_t* f0) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > B) return;
+  uint64_t lo = b < B ? 0 : items, hi = items;
+  if (b < B) {
+    const uint32_t r0 = b << S;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (pairs[mid].x < r0)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+  }
+  f0[b] = lo;
 }
 
 __global__ void k_deg_from_off(const uint64_t* off, uint32_t n, uint32_t* deg) {
@@ -1768,9 +1863,11 @@ void csr_part_hist(int mode, const uint32_t* a, const uint32_t* b, uint64_t item
                    uint32_t S, uint32_t B, uint32_t T, uint32_t* hist_t, uint32_t* err, hipStream_t s) {
   const size_t lds = sizeof(uint32_t) * B;
   if (mode == 0)
-    k_part_hist<0><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hist_t, err);
+    k_part_hist<0, false><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hist_t, err);
+  else if (mode == 2)
+    k_part_hist<0, true><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hist_t, err);
   else
-    k_part_hist<1><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, hist_t, err);
+    k_part_hist<1, false><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, hist_t, err);
 }
 void transpose_u32(const uint32_t* in, uint32_t rows, uint32_t cols, uint32_t* out, hipStream_t s) {
   if (rows && cols) k_transpose<uint32_t><<<dim3((cols + 63) / 64, (rows + 63) / 64), kBlock, 0, s>>>(in, rows, cols, out);
@@ -1784,14 +1881,36 @@ void csr_part_scatter(int mode, const uint32_t* a, const uint32_t* b, uint64_t i
   const size_t lds = sizeof(unsigned long long) * B;
   uint2* p = reinterpret_cast<uint2*>(part);
   if (mode == 0)
-    k_part_scatter<0><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hoff_t, p);
+    k_part_scatter<0, false><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hoff_t, p);
+  else if (mode == 2)
+    k_part_scatter<0, true><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hoff_t, p);
   else
-    k_part_scatter<1><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, hoff_t, p);
+    k_part_scatter<1, false><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, hoff_t, p);
 }
-void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, uint32_t B, uint32_t T, uint32_t S, uint32_t n,
-                     uint64_t* off, uint32_t* deg, uint32_t* adj, hipStream_t s) {
+void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, const uint32_t* fwd_pairs, const uint64_t* f0,
+                     uint32_t B, uint32_t T, uint32_t S, uint32_t n, uint64_t* off, uint32_t* deg, uint32_t* adj,
+                     hipStream_t s) {
   const size_t lds = sizeof(uint32_t) << S;
-  k_bucket_rows<<<B, kBucketBlock, lds, s>>>(reinterpret_cast<const uint2*>(part), hoff, T, S, n, off, deg, adj);
+  k_bucket_rows<<<B, kBucketBlock, lds, s>>>(reinterpret_cast<const uint2*>(part), hoff,
+                                             reinterpret_cast<const uint2*>(fwd_pairs), f0, T, S, n, off, deg, adj);
+}
+void csr_sorted_check(int mode, const uint32_t* keys, uint64_t items, uint32_t n, uint64_t* hpos, uint64_t* tpos,
+                      uint32_t* unsorted, uint32_t* err, hipStream_t s) {
+  if (items < 2 && !hpos) return;
+  const int grid = grid_for(items, 4 * kBlock);
+  if (mode == 0)
+    k_sorted_runs<2><<<grid, kBlock, 0, s>>>(keys, items, n, nullptr, nullptr, unsorted, err);
+  else
+    k_sorted_runs<1><<<grid, kBlock, 0, s>>>(keys, items, n, hpos, tpos, unsorted, err);
+}
+void csr_deg_from_runs(const uint64_t* hpos, const uint64_t* tpos, uint32_t n, uint32_t* deg, hipStream_t s) {
+  k_deg_from_runs<<<grid_for(uint64_t(n) + 1), kBlock, 0, s>>>(hpos, tpos, n, deg);
+}
+void csr_copy_cols(const uint32_t* xs, uint64_t L, uint32_t n, uint32_t* adj, uint32_t* err, hipStream_t s) {
+  if (L) k_copy_cols<<<grid_for(L, 4 * kBlock), kBlock, 0, s>>>(xs, L, n, adj, err);
+}
+void csr_fwd_start(const uint32_t* pairs, uint64_t items, uint32_t S, uint32_t B, uint64_t* f0, hipStream_t s) {
+  k_fwd_start<<<grid_for(uint64_t(B) + 1), kBlock, 0, s>>>(reinterpret_cast<const uint2*>(pairs), items, S, B, f0);
 }
 void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStream_t s) {
   if (n) k_deg_from_off<<<grid_for(n), kBlock, 0, s>>>(off, n, deg);

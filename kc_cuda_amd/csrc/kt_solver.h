@@ -184,6 +184,7 @@ class Solver {
   void reset_drop_ratio() { sparse_drop_ratio_ = default_drop_ratio_; }
   void set_force_edge_count(bool v) { force_edge_count_ = v; }
   void set_force_atomic_build(bool v) { force_atomic_build_ = v; }
+  void set_sorted_build(bool v) { sorted_build_ = v; }
   void set_core_mode(int m) { core_mode_ = m; }  // 0 auto, 1 sparse, 2 dense
   // pass 1 on dense views: walk `walk_levels` levels, then probe thresholds
   void set_gallop(bool v, uint32_t walk_levels) {
@@ -274,6 +275,7 @@ class Solver {
   void build_from_pairs(const uint32_t* d_pairs, uint64_t m, uint32_t n);
   void build_from_lines(const uint32_t* d_xs, const uint32_t* d_ys, uint64_t L, uint32_t n);
   bool build_partitioned(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint32_t n);
+  bool build_sorted_lines(const uint32_t* xs, const uint32_t* ys, uint64_t L, uint32_t n);
   void solve_csr(kt_result* res);
   uint32_t read_u32(const uint32_t* dptr);
   uint64_t read_u64(const uint64_t* dptr);
@@ -304,6 +306,8 @@ class Solver {
   int lanes_count_ = 0, lanes_peel_ = 0;
   bool force_edge_count_ = false;
   bool force_atomic_build_ = false;
+  bool sorted_build_ = true;  // row-sorted inputs skip (part of) the scatter
+
   int core_mode_ = 0;
   DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_, dhist_, vq_, dec_out_;
   int vertex_mode_ = 0;        // see set_vertex_mode

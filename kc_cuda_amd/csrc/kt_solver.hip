@@ -242,7 +242,16 @@ void Solver::sort_pairs_u64(const uint64_t* k_in, uint64_t* k_out, const uint32_
 bool Solver::build_partitioned(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint32_t n) {
   uint32_t S = 0, B = 0;
   if (!csr_partition_plan(n, &S, &B)) return false;
-  const uint64_t entries = mode == 0 ? 2 * items : items;
+  // pairs sorted by their first id: the first entries of a bucket's rows lie
+  // in order in the input, so only the second entries are bucketed
+  bool fwd_sorted = false;
+  if (mode == 0 && sorted_build_ && items > 1) {
+    zero_u32(&ctl_->pad[0]);
+    csr_sorted_check(0, a, items, n, nullptr, nullptr, &ctl_->pad[0], &ctl_->err, stream_);
+    fwd_sorted = read_u32(&ctl_->pad[0]) == 0;
+  }
+  const int pmode = fwd_sorted ? 2 : mode;
+  const uint64_t entries = mode == 0 && !fwd_sorted ? 2 * items : items;  // bucketed entries
   // at most build_tiles_ tiles of at least 65536 items
   const uint64_t tiles = std::max<uint32_t>(1, build_tiles_);
   const uint64_t tile = std::max<uint64_t>(65536, (items + tiles - 1) / tiles);
@@ -255,16 +264,42 @@ bool Solver::build_partitioned(int mode, const uint32_t* a, const uint32_t* b, u
   uint32_t* hist_b = hist_t + bt;
   uint64_t* hoff_b = hoff_.as<uint64_t>();
   uint64_t* hoff_t = hoff_b + bt + 1;
+  uint64_t* f0 = fwd_sorted ? cursor_.as<uint64_t>() : nullptr;  // B + 1 <= n + 1 words
   fill_u32(hist_b + bt, 1, 0, stream_);
   zero_u32(&ctl_->err);
-  csr_part_hist(mode, a, b, items, tile, n, S, B, T, hist_t, &ctl_->err, stream_);
+  csr_part_hist(pmode, a, b, items, tile, n, S, B, T, hist_t, &ctl_->err, stream_);
   if (read_u32(&ctl_->err)) fail("edge list holds a vertex id >= n");
   transpose_u32(hist_t, T, B, hist_b, stream_);
   exclusive_scan_u32_to_u64(hist_b, hoff_b, uint32_t(bt + 1));
   transpose_u64(hoff_b, B, T, hoff_t, stream_);
-  csr_part_scatter(mode, a, b, items, tile, n, S, B, T, hoff_t, part_.as<uint32_t>(), stream_);
-  csr_bucket_rows(part_.as<uint32_t>(), hoff_b, B, T, S, n, off(), deg(), adj(), stream_);
-  KT_HIP(hipMemcpyAsync(off() + n, hoff_b + bt, sizeof(uint64_t), hipMemcpyDeviceToDevice, stream_));
+  if (fwd_sorted) csr_fwd_start(a, items, S, B, f0, stream_);
+  csr_part_scatter(pmode, a, b, items, tile, n, S, B, T, hoff_t, part_.as<uint32_t>(), stream_);
+  csr_bucket_rows(part_.as<uint32_t>(), hoff_b, fwd_sorted ? a : nullptr, f0, B, T, S, n, off(), deg(), adj(),
+                  stream_);
+  if (fwd_sorted)
+    fill_u64(off() + n, 1, 2 * items, stream_);
+  else
+    KT_HIP(hipMemcpyAsync(off() + n, hoff_b + bt, sizeof(uint64_t), hipMemcpyDeviceToDevice, stream_));
+  return true;
+}
+
+// Lines sorted by row (the reference's edge-list order): each row is a run
+// of the lines, so the offsets come from the runs' ends and adj is xs as it
+// is. Returns false (nothing built) when the rows are not sorted.
+bool Solver::build_sorted_lines(const uint32_t* xs, const uint32_t* ys, uint64_t L, uint32_t n) {
+  uint64_t* hpos = cursor_.as<uint64_t>();
+  uint64_t* tpos = off();
+  fill_u64(hpos, uint64_t(n) + 1, 0, stream_);
+  fill_u64(tpos, uint64_t(n) + 1, 0, stream_);
+  zero_u32(&ctl_->pad[0]);
+  zero_u32(&ctl_->err);
+  csr_sorted_check(1, ys, L, n, hpos, tpos, &ctl_->pad[0], &ctl_->err, stream_);
+  if (read_u32(&ctl_->pad[0]) != 0) return false;
+  if (read_u32(&ctl_->err)) fail("edge list holds a vertex id >= n");
+  csr_deg_from_runs(hpos, tpos, n, deg(), stream_);
+  exclusive_scan_u32_to_u64(deg(), off(), n + 1);
+  csr_copy_cols(xs, L, n, adj(), &ctl_->err, stream_);
+  if (read_u32(&ctl_->err)) fail("edge list holds a vertex id >= n");
   return true;
 }
 
@@ -304,6 +339,7 @@ void Solver::build_from_lines(const uint32_t* d_xs, const uint32_t* d_ys, uint64
   off_.ensure(sizeof(uint64_t) * (size_t(n) + 1));
   cursor_.ensure(sizeof(uint64_t) * (size_t(n) + 1));
   adj_.ensure(sizeof(uint32_t) * std::max<uint64_t>(L, 1));
+  if (!force_atomic_build_ && sorted_build_ && build_sorted_lines(d_xs, d_ys, L, n)) return;
   if (!force_atomic_build_ && build_partitioned(1, d_xs, d_ys, L, n)) return;
   fill_u32(deg(), uint64_t(n) + 1, 0, stream_);
   zero_u32(&ctl_->err);

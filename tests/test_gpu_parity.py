@@ -166,29 +166,80 @@ def test_compaction_runs_on_rmat16(ctx):
     np.testing.assert_array_equal(ctx.truss_edges(), canon_pairs(ref.truss))
 
 
-@pytest.mark.parametrize("atomic_build", [False, True])
+@pytest.mark.parametrize("build", ["auto", "atomic", "unsorted"])
+@pytest.mark.parametrize("order", ["sorted", "shuffled", "flipped"])
 @pytest.mark.parametrize("lines", [False, True])
-def test_csr_matches_oracle(ctx, atomic_build, lines):
+def test_csr_matches_oracle(ctx, build, order, lines):
+    """CSR rows (as sets) and offsets against the oracle's, for inputs in the
+    reference's order (lines sorted by row, pairs sorted: the builds that
+    read rows as runs), shuffled (the bucketed build), and, for pairs, with
+    a random half flipped to (v, u) and then sorted by first id (sorted by
+    first id but not canonical)."""
     from kc_cuda_amd import gen
     O = _oracle()
     pairs, n = gen.rmat(12, 8, 0.57, 0.19, 0.19, True, 0, 5)
-    print(f"[gpu] csr atomic_build={atomic_build} lines={lines}", flush=True)
+    rng = np.random.default_rng(9)
+    if order == "flipped":
+        if lines:
+            pytest.skip("lines hold both directions already")
+        flip = rng.random(len(pairs)) < 0.5
+        pairs = pairs.copy()
+        pairs[flip] = pairs[flip][:, ::-1]
+        pairs = pairs[np.argsort(pairs[:, 0], kind="stable")]
+    print(f"[gpu] csr build={build} order={order} lines={lines}", flush=True)
     try:
-        ctx.tune(force_atomic_build=atomic_build)
+        ctx.tune(force_atomic_build=build == "atomic", unsorted_build=build == "unsorted")
         if lines:
             xs, ys = O.lines_from_pairs(pairs)
+            if order == "shuffled":
+                p = rng.permutation(len(xs))
+                xs, ys = np.ascontiguousarray(xs[p]), np.ascontiguousarray(ys[p])
             ctx.solve_lines(xs, ys, n)
         else:
-            ctx.solve_pairs(pairs, n)
+            q = pairs[rng.permutation(len(pairs))] if order == "shuffled" else pairs
+            ctx.solve_pairs(np.ascontiguousarray(q), n)
         off, adj = ctx.last_csr(n, 2 * len(pairs))
     finally:
         ctx.tune()
-    ooff, oadj, _ = O.csr_from_pairs(pairs, n)
+    ooff, oadj, _ = O.csr_from_pairs(canon_pairs(pairs), n)
     np.testing.assert_array_equal(off, ooff)
     for v in range(n):  # rows are sets: the device scatter order is free
         a = np.sort(adj[off[v]:off[v + 1]])
         b = np.sort(oadj[ooff[v]:ooff[v + 1]])
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("lines", [False, True])
+def test_sorted_build_multi_bucket(ctx, lines):
+    """Row-sorted inputs over many buckets (n = 2^18: 256 buckets of 1024
+    rows, with empty rows at both ends and inside) give the bucketed build's
+    CSR, and the same solve."""
+    from kc_cuda_amd import gen
+    O = _oracle()
+    pairs, _ = gen.rmat(16, 8, 0.57, 0.19, 0.19, True, 0, 12)
+    n = 1 << 18
+    pairs = canon_pairs((pairs.astype(np.uint64) * 3 + 7).astype(np.uint32))  # ids spread out, gaps
+    print(f"[gpu] sorted multi-bucket lines={lines} m={len(pairs)}", flush=True)
+    got = {}
+    for build in ("auto", "unsorted"):
+        try:
+            ctx.tune(unsorted_build=build == "unsorted")
+            if lines:
+                xs, ys = O.lines_from_pairs(pairs)
+                r = ctx.solve_lines(xs, ys, n)
+            else:
+                r = ctx.solve_pairs(pairs, n)
+            off, adj = ctx.last_csr(n, 2 * len(pairs))
+        finally:
+            ctx.tune()
+        rows = np.repeat(np.arange(n, dtype=np.uint64), np.diff(off.astype(np.int64)))
+        key = np.sort((rows << np.uint64(32)) | adj.astype(np.uint64))
+        got[build] = (off, key, r.kmax, r.truss_edges)
+    np.testing.assert_array_equal(got["auto"][0], got["unsorted"][0])
+    np.testing.assert_array_equal(got["auto"][1], got["unsorted"][1])
+    assert got["auto"][2:] == got["unsorted"][2:]
+    ooff, _, _ = O.csr_from_pairs(pairs, n)
+    np.testing.assert_array_equal(got["auto"][0], ooff)
 
 
 def test_tsv_path_and_cli(ctx, tmp_path):
