@@ -477,10 +477,21 @@ def ingest_line(args) -> dict:
             del hx, hy, xs, ys
             for _ in range(args.warmup):
                 ctx.solve_tsv(tsv)
-            times = []
+            times, builds, solves = [], [], []
             for _ in range(args.steps):
                 r, parse_ms = ctx.solve_tsv(tsv)
                 times.append(parse_ms / 1e3)
+                builds.append(r.ms_build)
+                solves.append(r.ms_total)
+            # the same solve with the bucketed CSR build the lines would take
+            # if they were not sorted by row
+            ctx.tune(unsorted_build=True)
+            try:
+                rb, _ = ctx.solve_tsv(tsv)
+            finally:
+                ctx.tune()
+            if (rb.kmax, rb.truss_edges) != (r.kmax, r.truss_edges):
+                raise RuntimeError("bucketed and run builds disagree")
     t = statistics.median(times)
     log(f"[bench] ingest: file to line arrays in HBM {t * 1e3:.1f} ms (host parser {t_host * 1e3:.0f} ms); "
         f"kmax={r.kmax} |T|={r.truss_edges}")
@@ -491,6 +502,10 @@ def ingest_line(args) -> dict:
             "config": {"workload": args.workload, "text_bytes": nbytes, "lines": 2 * m,
                        "timed": "file read, H2D copy and device parse, to line arrays in HBM",
                        "host_parser_ms": t_host * 1e3,
+                       "solve_ms": statistics.median(solves),
+                       "csr_build_ms": statistics.median(builds),
+                       "csr_build_ms_bucketed": rb.ms_build,
+                       "csr_build": "lines sorted by row: offsets from their runs, columns copied",
                        "kmax": r.kmax, "truss_edges": int(r.truss_edges)},
             "roofline": None, "cpu_baseline": None}
 

@@ -307,6 +307,8 @@ class Solver {
   bool force_edge_count_ = false;
   bool force_atomic_build_ = false;
   bool sorted_build_ = true;  // row-sorted inputs skip (part of) the scatter
+  int64_t walk_window_ = 32;   // KT_WALK_WINDOW: pass-1 thresholds left when probes give way to walking
+  int64_t gallop_growth_ = 2;  // KT_GALLOP_GROWTH: factor of the pass-1 probe steps
 
   int core_mode_ = 0;
   DevBuf bits0_, bits_, fbits_, pref0_, eidm_, heavy_, vflag_, vmap_, roff_, dhist_, vq_, dec_out_;

@@ -121,6 +121,8 @@ void Solver::init_common() {
   if (const char* v = std::getenv("KT_DENSE_DROP_FIXED")) dense_drop_fixed_ = std::atof(v);
   if (const char* v = std::getenv("KT_SPARSE_DROP_RATIO")) sparse_drop_ratio_ = std::atof(v);
   if (const char* v = std::getenv("KT_TRACE")) trace_ = v[0] == '1';
+  if (const char* v = std::getenv("KT_WALK_WINDOW")) walk_window_ = std::max<int64_t>(1, std::atoll(v));
+  if (const char* v = std::getenv("KT_GALLOP_GROWTH")) gallop_growth_ = std::max<int64_t>(2, std::atoll(v));
   if (const char* v = std::getenv("KT_EIDM_BUDGET_MB")) eidm_budget_ = uint64_t(std::atoll(v)) << 20;
   if (const char* v = std::getenv("KT_WALK_LEVELS")) walk_levels_ = std::max(1, std::atoi(v));
   if (const char* v = std::getenv("KT_BUILD_TILES")) build_tiles_ = uint32_t(std::max(1, std::atoi(v)));

@@ -753,7 +753,7 @@ TrussOutcome Solver::run_levels(CoreView* cv, int32_t thr, uint64_t alive, uint3
 // deleted on the way carries a threshold <= the floor. (The reference's
 // speculative k-steps with backups, src/kc.cu:1189-1196, 1252-1272.)
 TrussOutcome Solver::truss_gallop(CoreView* cv, int64_t empty_from) {
-  constexpr int64_t kWalkWindow = 32;  // thresholds left when probing gives way to walking
+  const int64_t kWalkWindow = walk_window_;  // thresholds left when probing gives way to walking
   TrussOutcome out;
   if (cv->mc == 0) return out;
   SearchRule walk{};
@@ -836,7 +836,7 @@ TrussOutcome Solver::truss_gallop(CoreView* cv, int64_t empty_from) {
       out.kmax = p.kmax;
       out.count = p.count;
       out.floor_thr = p.floor_thr;
-      step *= 2;
+      step *= gallop_growth_;
     } else {
       restore();
       hi = t;
