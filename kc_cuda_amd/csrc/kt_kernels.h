@@ -22,18 +22,15 @@ void degrees_from_offsets(const uint64_t* off, uint32_t n, uint32_t* deg, hipStr
 
 // Partitioned build: entries bucketed by row >> S (B buckets, T tiles of
 // `tile` items), then rows built per bucket in LDS. mode 0: a = interleaved
-// pairs (both directions); mode 1: a = xs (columns), b = ys (rows); mode 2
-// (csr_part_scatter): pairs, second entries (row v, column u) only.
+// pairs (both directions); mode 1: a = xs (columns), b = ys (rows); mode 2:
+// pairs, second entries (row v, column u) only.
 // hist_t has T*B entries (tile-major, one row per tile); its transpose is
 // scanned bucket-major into hoff (B*T+1), whose tile-major transpose hoff_t
 // gives each tile its write cursors.
 bool csr_partition_plan(uint32_t n, uint32_t* S, uint32_t* B);
-// csr_part_hist, mode 0 with hist_s: hist_s (T*B) also counts the second
-// entries alone and *unsorted |= 1 unless the pairs' first ids never
-// decrease (row-sorted pairs then take mode 2 for the scatter).
 void csr_part_hist(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile,
-                   uint32_t n, uint32_t S, uint32_t B, uint32_t T, uint32_t* hist_t, uint32_t* hist_s,
-                   uint32_t* err, uint32_t* unsorted, hipStream_t s);
+                   uint32_t n, uint32_t S, uint32_t B, uint32_t T, uint32_t* hist_t, uint32_t* err,
+                   hipStream_t s);
 void transpose_u32(const uint32_t* in, uint32_t rows, uint32_t cols, uint32_t* out, hipStream_t s);
 void transpose_u64(const uint64_t* in, uint32_t rows, uint32_t cols, uint64_t* out, hipStream_t s);
 void csr_part_scatter(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile,
@@ -45,14 +42,15 @@ void csr_part_scatter(int mode, const uint32_t* a, const uint32_t* b, uint64_t i
 void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, const uint32_t* fwd_pairs, const uint64_t* f0,
                      uint32_t B, uint32_t T, uint32_t S, uint32_t n, uint64_t* off, uint32_t* deg, uint32_t* adj,
                      hipStream_t s);
-// Lines sorted by row. csr_sorted_runs: *unsorted |= 1 unless ys never
-// decreases; hpos[r] / tpos[r] = first / one past the last line of row r
-// (fill both with 0 first); *err |= 1 for a row >= n. csr_deg_from_runs:
+// Row-sorted inputs. csr_sorted_check: *unsorted |= 1 unless the rows are
+// non-decreasing (mode 0: pairs by first id; mode 1: lines by ys); for lines
+// also hpos[r] / tpos[r] = first / one past the last line of row r (fill
+// both with 0 first) and *err |= 1 for a row >= n. csr_deg_from_runs:
 // deg[0..n] from them (deg[n] = 0). csr_copy_cols: adj = xs (*err |= 1 for a
-// column >= n). Pairs sorted by first id: csr_fwd_start: f0[b] = first pair
-// with first id >= b << S, f0[B] = items.
-void csr_sorted_runs(const uint32_t* ys, uint64_t items, uint32_t n, uint64_t* hpos, uint64_t* tpos,
-                     uint32_t* unsorted, uint32_t* err, hipStream_t s);
+// column >= n). csr_fwd_start: f0[b] = first pair with first id >= b << S,
+// f0[B] = items.
+void csr_sorted_check(int mode, const uint32_t* keys, uint64_t items, uint32_t n, uint64_t* hpos, uint64_t* tpos,
+                      uint32_t* unsorted, uint32_t* err, hipStream_t s);
 void csr_deg_from_runs(const uint64_t* hpos, const uint64_t* tpos, uint32_t n, uint32_t* deg, hipStream_t s);
 void csr_copy_cols(const uint32_t* xs, uint64_t L, uint32_t n, uint32_t* adj, uint32_t* err, hipStream_t s);
 void csr_fwd_start(const uint32_t* pairs, uint64_t items, uint32_t S, uint32_t B, uint64_t* f0, hipStream_t s);

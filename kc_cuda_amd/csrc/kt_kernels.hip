@@ -147,30 +147,25 @@ __device__ __forceinline__ LaneRun lane_run(uint32_t key, bool active) {
   return r;
 }
 
-// kSplit (pairs only): hist_t counts both entries of each pair and hist_s
-// only the second ones (row v, column u), and *unsorted |= 1 unless the
-// pairs' first ids never decrease -- the scatter then moves either both
-// entries or, for row-sorted pairs, only the second ones (kBwd below), whose
-// first entries k_bucket_rows places straight from the input.
-template <int Mode, bool kSplit>
+// kBwd (pairs only): count and scatter only each pair's second entry (row v,
+// column u); the first entries of row-sorted pairs are placed in order by
+// k_bucket_rows_sorted instead.
+template <int Mode, bool kBwd>
 __global__ __launch_bounds__(kBlock) void k_part_hist(EntrySrc<Mode> src, uint64_t tile, uint32_t n,
-                                                      uint32_t S, uint32_t B, uint32_t* hist_t, uint32_t* hist_s,
-                                                      uint32_t* err, uint32_t* unsorted) {
+                                                      uint32_t S, uint32_t B, uint32_t* hist_t,
+                                                      uint32_t* err) {
   extern __shared__ uint32_t h[];
-  uint32_t* hs = h + B;  // kSplit: second entries only
   const uint32_t t = blockIdx.x;
-  for (uint32_t i = threadIdx.x; i < (kSplit ? 2 * B : B); i += blockDim.x) h[i] = 0;
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) h[i] = 0;
   __syncthreads();
   const uint64_t lo = uint64_t(t) * tile;
   const uint64_t hi = min(src.items, lo + tile);
-  bool bad = false;
   for (uint64_t i0 = lo; i0 < hi; i0 += blockDim.x) {  // block-uniform trip count: every lane reaches the shuffles
     const uint64_t i = i0 + threadIdx.x;
     uint32_t rows[2] = {0, 0}, cols[2] = {0, 0};
     const bool in = i < hi;
     if (in) src.get(i, rows, cols);
-    if (kSplit && in && i + 1 < src.items) bad |= reinterpret_cast<const uint2*>(src.a)[i + 1].x < rows[0];
-    for (int k = 0; k < src.per_item(); ++k) {
+    for (int k = kBwd ? 1 : 0; k < src.per_item(); ++k) {
       bool ok = in;
       if (in && (rows[k] >= n || cols[k] >= n)) {
         atomicOr(err, 1u);
@@ -178,18 +173,11 @@ __global__ __launch_bounds__(kBlock) void k_part_hist(EntrySrc<Mode> src, uint64
       }
       const uint32_t b = rows[k] >> S;
       const LaneRun r = lane_run(b, ok);
-      if (r.is_head) {
-        atomicAdd(&h[b], r.len);
-        if (kSplit && k == 1) atomicAdd(&hs[b], r.len);
-      }
+      if (r.is_head) atomicAdd(&h[b], r.len);
     }
   }
-  if (kSplit && __syncthreads_or(bad) && threadIdx.x == 0) atomicOr(unsorted, 1u);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) {
-    hist_t[uint64_t(t) * B + i] = h[i];
-    if (kSplit) hist_s[uint64_t(t) * B + i] = hs[i];
-  }
+  for (uint32_t i = threadIdx.x; i < B; i += blockDim.x) hist_t[uint64_t(t) * B + i] = h[i];
 }
 
 // out[c * rows + r] = in[r * cols + c], through 64 x 64 tiles in LDS (a
@@ -211,7 +199,6 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const T* __restrict__ in, 
   }
 }
 
-// kBwd (pairs only): scatter only each pair's second entry (row v, column u).
 template <int Mode, bool kBwd>
 __global__ __launch_bounds__(kBlock) void k_part_scatter(EntrySrc<Mode> src, uint64_t tile, uint32_t n,
                                                          uint32_t S, uint32_t B,
@@ -336,29 +323,33 @@ __global__ __launch_bounds__(kBucketBlock) void k_bucket_rows(const uint2* __res
 // sorted by their first id. Such inputs skip the scatter of the lines (their
 // rows are runs already) or of the pairs' first entries.
 
-// Lines: *unsorted becomes non-zero when ys[i] > ys[i + 1] for some i; each
-// run of equal rows r < n records hpos[r] = its first index and tpos[r] =
-// its last index + 1 (valid when the rows are sorted; rows without lines
-// keep what the caller filled); a row >= n sets *err.
-__global__ __launch_bounds__(kBlock) void k_sorted_runs(const uint32_t* __restrict__ ys, uint64_t items, uint32_t n,
-                                                        uint64_t* hpos, uint64_t* tpos, uint32_t* unsorted,
-                                                        uint32_t* err) {
+// *unsorted becomes non-zero when keys[i * Stride] > keys[(i + 1) * Stride]
+// for some i. With hpos / tpos (lines), each run of equal rows r < n also
+// records hpos[r] = its first index and tpos[r] = its last index + 1 (valid
+// when the keys are sorted; rows without lines keep what the caller
+// filled); a row >= n sets *err.
+template <int Stride>
+__global__ __launch_bounds__(kBlock) void k_sorted_runs(const uint32_t* __restrict__ keys, uint64_t items,
+                                                        uint32_t n, uint64_t* hpos, uint64_t* tpos,
+                                                        uint32_t* unsorted, uint32_t* err) {
   bool bad = false, big = false;
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < items; i += stride) {
-    const uint32_t k = ys[i];
+    const uint32_t k = keys[i * Stride];
     const bool has_next = i + 1 < items;
-    const uint32_t nx = has_next ? ys[i + 1] : 0u;
+    const uint32_t nx = has_next ? keys[(i + 1) * Stride] : 0u;
     bad |= has_next && nx < k;
-    if (k >= n) {
-      big = true;
-      continue;
+    if (hpos) {
+      if (k >= n) {
+        big = true;
+        continue;
+      }
+      if (i == 0 || keys[(i - 1) * Stride] != k) hpos[k] = i;
+      if (!has_next || nx != k) tpos[k] = i + 1;
     }
-    if (i == 0 || ys[i - 1] != k) hpos[k] = i;
-    if (!has_next || nx != k) tpos[k] = i + 1;
   }
   if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(unsorted, 1u);
-  if (__syncthreads_or(big) && threadIdx.x == 0) atomicOr(err, 1u);
+  if (hpos && __syncthreads_or(big) && threadIdx.x == 0) atomicOr(err, 1u);
 }
 
 // deg[r] = tpos[r] - hpos[r] for r < n, deg[n] = 0.
@@ -1869,20 +1860,14 @@ bool csr_partition_plan(uint32_t n, uint32_t* S, uint32_t* B) {
   return false;
 }
 void csr_part_hist(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint64_t tile, uint32_t n,
-                   uint32_t S, uint32_t B, uint32_t T, uint32_t* hist_t, uint32_t* hist_s, uint32_t* err,
-                   uint32_t* unsorted, hipStream_t s) {
-  if (mode == 0) {
-    const size_t lds = sizeof(uint32_t) * (hist_s ? 2 * B : B);
-    if (hist_s)
-      k_part_hist<0, true><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hist_t, hist_s, err,
-                                                  unsorted);
-    else
-      k_part_hist<0, false><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hist_t, nullptr, err,
-                                                   nullptr);
-  } else {
-    k_part_hist<1, false><<<T, kBlock, sizeof(uint32_t) * B, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, hist_t,
-                                                                   nullptr, err, nullptr);
-  }
+                   uint32_t S, uint32_t B, uint32_t T, uint32_t* hist_t, uint32_t* err, hipStream_t s) {
+  const size_t lds = sizeof(uint32_t) * B;
+  if (mode == 0)
+    k_part_hist<0, false><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hist_t, err);
+  else if (mode == 2)
+    k_part_hist<0, true><<<T, kBlock, lds, s>>>(EntrySrc<0>{a, b, items}, tile, n, S, B, hist_t, err);
+  else
+    k_part_hist<1, false><<<T, kBlock, lds, s>>>(EntrySrc<1>{a, b, items}, tile, n, S, B, hist_t, err);
 }
 void transpose_u32(const uint32_t* in, uint32_t rows, uint32_t cols, uint32_t* out, hipStream_t s) {
   if (rows && cols) k_transpose<uint32_t><<<dim3((cols + 63) / 64, (rows + 63) / 64), kBlock, 0, s>>>(in, rows, cols, out);
@@ -1909,9 +1894,14 @@ void csr_bucket_rows(const uint32_t* part, const uint64_t* hoff, const uint32_t*
   k_bucket_rows<<<B, kBucketBlock, lds, s>>>(reinterpret_cast<const uint2*>(part), hoff,
                                              reinterpret_cast<const uint2*>(fwd_pairs), f0, T, S, n, off, deg, adj);
 }
-void csr_sorted_runs(const uint32_t* ys, uint64_t items, uint32_t n, uint64_t* hpos, uint64_t* tpos,
-                     uint32_t* unsorted, uint32_t* err, hipStream_t s) {
-  k_sorted_runs<<<grid_for(items, 4 * kBlock), kBlock, 0, s>>>(ys, items, n, hpos, tpos, unsorted, err);
+void csr_sorted_check(int mode, const uint32_t* keys, uint64_t items, uint32_t n, uint64_t* hpos, uint64_t* tpos,
+                      uint32_t* unsorted, uint32_t* err, hipStream_t s) {
+  if (items < 2 && !hpos) return;
+  const int grid = grid_for(items, 4 * kBlock);
+  if (mode == 0)
+    k_sorted_runs<2><<<grid, kBlock, 0, s>>>(keys, items, n, nullptr, nullptr, unsorted, err);
+  else
+    k_sorted_runs<1><<<grid, kBlock, 0, s>>>(keys, items, n, hpos, tpos, unsorted, err);
 }
 void csr_deg_from_runs(const uint64_t* hpos, const uint64_t* tpos, uint32_t n, uint32_t* deg, hipStream_t s) {
   k_deg_from_runs<<<grid_for(uint64_t(n) + 1), kBlock, 0, s>>>(hpos, tpos, n, deg);

@@ -244,34 +244,34 @@ void Solver::sort_pairs_u64(const uint64_t* k_in, uint64_t* k_out, const uint32_
 bool Solver::build_partitioned(int mode, const uint32_t* a, const uint32_t* b, uint64_t items, uint32_t n) {
   uint32_t S = 0, B = 0;
   if (!csr_partition_plan(n, &S, &B)) return false;
+  // pairs sorted by their first id: the first entries of a bucket's rows lie
+  // in order in the input, so only the second entries are bucketed
+  bool fwd_sorted = false;
+  if (mode == 0 && sorted_build_ && items > 1) {
+    zero_u32(&ctl_->pad[0]);
+    csr_sorted_check(0, a, items, n, nullptr, nullptr, &ctl_->pad[0], &ctl_->err, stream_);
+    fwd_sorted = read_u32(&ctl_->pad[0]) == 0;
+  }
+  const int pmode = fwd_sorted ? 2 : mode;
+  const uint64_t entries = mode == 0 && !fwd_sorted ? 2 * items : items;  // bucketed entries
   // at most build_tiles_ tiles of at least 65536 items
   const uint64_t tiles = std::max<uint32_t>(1, build_tiles_);
   const uint64_t tile = std::max<uint64_t>(65536, (items + tiles - 1) / tiles);
   const uint32_t T = uint32_t(std::max<uint64_t>(1, (items + tile - 1) / tile));
   const uint64_t bt = uint64_t(B) * T;
-  // pairs: the histogram pass also counts the second entries alone and checks
-  // whether the pairs are sorted by their first id; then the first entries of
-  // a bucket's rows lie in order in the input and only the second ones move
-  const bool split = mode == 0 && sorted_build_ && items > 1;
-  hist_.ensure(sizeof(uint32_t) * ((split ? 3 : 2) * bt + 1));  // tile-major counts, bucket-major, [second only]
-  hoff_.ensure(sizeof(uint64_t) * (2 * bt + 1));                 // bucket-major offsets, then tile-major
+  hist_.ensure(sizeof(uint32_t) * (2 * bt + 1));   // tile-major counts, then bucket-major
+  hoff_.ensure(sizeof(uint64_t) * (2 * bt + 1));   // bucket-major offsets, then tile-major
+  part_.ensure(sizeof(uint32_t) * 2 * std::max<uint64_t>(entries, 1));
   uint32_t* hist_t = hist_.as<uint32_t>();
   uint32_t* hist_b = hist_t + bt;
-  uint32_t* hist_s = split ? hist_b + bt + 1 : nullptr;
   uint64_t* hoff_b = hoff_.as<uint64_t>();
   uint64_t* hoff_t = hoff_b + bt + 1;
+  uint64_t* f0 = fwd_sorted ? cursor_.as<uint64_t>() : nullptr;  // B + 1 <= n + 1 words
   fill_u32(hist_b + bt, 1, 0, stream_);
   zero_u32(&ctl_->err);
-  if (split) zero_u32(&ctl_->pad[0]);
-  csr_part_hist(mode, a, b, items, tile, n, S, B, T, hist_t, hist_s, &ctl_->err, &ctl_->pad[0], stream_);
-  const Ctl hc = read_ctl();
-  if (hc.err) fail("edge list holds a vertex id >= n");
-  const bool fwd_sorted = split && hc.pad[0] == 0;
-  const int pmode = fwd_sorted ? 2 : mode;
-  const uint64_t entries = mode == 0 && !fwd_sorted ? 2 * items : items;  // bucketed entries
-  part_.ensure(sizeof(uint32_t) * 2 * std::max<uint64_t>(entries, 1));
-  uint64_t* f0 = fwd_sorted ? cursor_.as<uint64_t>() : nullptr;  // B + 1 <= n + 1 words
-  transpose_u32(fwd_sorted ? hist_s : hist_t, T, B, hist_b, stream_);
+  csr_part_hist(pmode, a, b, items, tile, n, S, B, T, hist_t, &ctl_->err, stream_);
+  if (read_u32(&ctl_->err)) fail("edge list holds a vertex id >= n");
+  transpose_u32(hist_t, T, B, hist_b, stream_);
   exclusive_scan_u32_to_u64(hist_b, hoff_b, uint32_t(bt + 1));
   transpose_u64(hoff_b, B, T, hoff_t, stream_);
   if (fwd_sorted) csr_fwd_start(a, items, S, B, f0, stream_);
@@ -295,10 +295,9 @@ bool Solver::build_sorted_lines(const uint32_t* xs, const uint32_t* ys, uint64_t
   fill_u64(tpos, uint64_t(n) + 1, 0, stream_);
   zero_u32(&ctl_->pad[0]);
   zero_u32(&ctl_->err);
-  csr_sorted_runs(ys, L, n, hpos, tpos, &ctl_->pad[0], &ctl_->err, stream_);
-  const Ctl hc = read_ctl();
-  if (hc.pad[0] != 0) return false;
-  if (hc.err) fail("edge list holds a vertex id >= n");
+  csr_sorted_check(1, ys, L, n, hpos, tpos, &ctl_->pad[0], &ctl_->err, stream_);
+  if (read_u32(&ctl_->pad[0]) != 0) return false;
+  if (read_u32(&ctl_->err)) fail("edge list holds a vertex id >= n");
   csr_deg_from_runs(hpos, tpos, n, deg(), stream_);
   exclusive_scan_u32_to_u64(deg(), off(), n + 1);
   csr_copy_cols(xs, L, n, adj(), &ctl_->err, stream_);
