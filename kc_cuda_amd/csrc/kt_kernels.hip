@@ -874,11 +874,15 @@ __global__ void k_core_reopen_above(uint32_t* lvl, uint32_t n, uint32_t k) {
 // degree < k (lvl = k), and the queue is drained in steps separated by
 // barriers, one wave per queued vertex: every neighbour loses one degree, and
 // the decrement that takes it from k to k - 1 queues it (an LDS atomic, so
-// no vertex is queued twice). When a threshold removes every alive vertex,
-// the vertices it removed are the max core: they are re-opened (lvl = kNone)
-// with the degrees they had when that threshold started. A removed vertex's LDS
-// degree carries bit 31; afterwards lvl[v] = 1 + the core number of every
-// vertex removed here, so every threshold from k0 up is an exact cut.
+// no vertex is queued twice). When a threshold empties the level, the
+// vertices alive at its start are the max core: the ones it removed are
+// re-opened (lvl = kNone), and all have the degrees they had when that
+// threshold started. A non-empty k-core has at least k + 1 vertices, so a
+// threshold is known to empty the level as soon as fewer than k + 1 are left,
+// and its cascade stops there (on a dense max core, most of its rows are
+// then never walked). A removed vertex's LDS degree carries bit 31;
+// afterwards lvl[v] = 1 + the core number of every vertex removed here, so
+// every threshold from k0 up is an exact cut.
 constexpr uint32_t kDecompBlock = 1024;
 constexpr uint32_t kDecompMaxN = 16000;   // LDS degrees: 64,000 B
 constexpr uint32_t kDecompUnroll = 8;     // row slots a lane loads before its atomics
@@ -908,7 +912,8 @@ __global__ __launch_bounds__(kDecompBlock) void k_core_decompose(const uint64_t*
   __syncthreads();
   uint32_t k = max(k0, s_min + 1);  // every alive degree is >= s_min: the first removing threshold
   uint32_t start = 0;               // queue position where threshold k's removals begin
-  bool done = s_alive == 0;
+  uint32_t alive0 = s_alive;        // vertices alive when threshold k starts
+  bool done = alive0 == 0;
   while (!done) {
     // the threshold's first vertices; the alive degrees are kept in deg, so
     // the threshold that empties the level leaves the max core's degrees there
@@ -925,8 +930,13 @@ __global__ __launch_bounds__(kDecompBlock) void k_core_decompose(const uint64_t*
     __syncthreads();
     // drain: one wave per queued vertex, in steps separated by barriers
     uint32_t head = start;
+    bool empties = false;  // fewer than k + 1 vertices left: the k-core is empty
     for (;;) {
       const uint32_t tail = s_tail;
+      if (alive0 - (tail - start) < k + 1) {
+        empties = true;
+        break;
+      }
       if (head == tail) break;
       for (uint32_t i = head + wid; i < tail; i += kWaves) {
         const uint32_t v = q[i];
@@ -950,6 +960,7 @@ __global__ __launch_bounds__(kDecompBlock) void k_core_decompose(const uint64_t*
       head = tail;
       __syncthreads();
     }
+    if (empties) break;  // s_tail is final: the check above follows a barrier
     // s_tail is final for this threshold
     const uint32_t removed = s_tail - start;
     __syncthreads();
@@ -968,16 +979,19 @@ __global__ __launch_bounds__(kDecompBlock) void k_core_decompose(const uint64_t*
       __syncthreads();
       start = s_tail;
       k = s_min + 1;
+      alive0 = s_alive;
       __syncthreads();
     }
   }
-  // the last threshold's removals (queue [start, s_tail)) are the max core;
-  // deg holds their degrees at that threshold's start, i.e. within the core
+  // the vertices alive when the last threshold started are the max core:
+  // its removals (queue [start, s_tail)) are re-opened, the others were never
+  // removed; deg holds all their degrees at that threshold's start, i.e.
+  // within the core
   const uint32_t tail = s_tail;
   for (uint32_t i = start + tid; i < tail; i += kDecompBlock) lvl[q[i]] = kNone;
   if (tid == 0) {
-    out[0] = tail > start ? k - 1 : 0;  // the max core's k (0: nothing was alive)
-    out[1] = tail - start;              // its vertices
+    out[0] = alive0 ? k - 1 : 0;  // the max core's k (0: nothing was alive)
+    out[1] = alive0;              // its vertices
   }
 }
 
