@@ -76,11 +76,13 @@ struct Ctl {
   uint32_t vq_done;
   uint32_t vq_chunks;
   uint32_t vq_err;
+  uint32_t vq_removed;   // vertices queued so far (a lower bound while waves run)
 };
 
 // ctl->vq_err values
 constexpr uint32_t kVqOverflow = 1;  // an item past the queue's capacity
 constexpr uint32_t kVqStall = 2;     // a wave waited past its spin limit
+constexpr uint32_t kVqEmpty = 8;     // vertex cascade: too few vertices left for a k-core (not an error)
 
 // ctl->err bit: a level ended with live edges but no live support (or the
 // reverse); the edge-peel bookkeeping is inconsistent

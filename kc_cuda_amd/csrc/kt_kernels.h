@@ -75,9 +75,11 @@ void core_round(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t* 
 // vertices were removed (lvl = k), and ctl->vq_err is non-zero if the queue
 // overflowed or a wave gave up waiting. Measurement as for core_round.
 uint64_t cascade_capacity(uint32_t n, uint64_t slots);
+// stop_at != 0: leave as soon as stop_at vertices are queued (the k-core is
+// then empty; ctl->vq_err gets kVqEmpty and the level is left part-peeled).
 void core_cascade(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t k, uint32_t* lvl, uint32_t* deg,
                   unsigned long long* q, uint32_t cap, Ctl* ctl, unsigned long long* stats, hipEvent_t ev0,
-                  hipEvent_t ev1, hipStream_t s);
+                  hipEvent_t ev1, hipStream_t s, uint32_t stop_at = 0);
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl,
                         uint32_t n, uint32_t* deg, HeavyList hl, hipStream_t s);
 // lvl[v] = (lvl[v] > k) ? kNone : (lvl[v] == kNone ? kNone : k) -- used to

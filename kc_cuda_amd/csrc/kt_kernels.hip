@@ -630,6 +630,7 @@ __global__ void k_vcascade_start(Ctl* ctl) {
   ctl->vq_done = 0;
   ctl->vq_chunks = 0;
   ctl->vq_err = 0;
+  ctl->vq_removed = 0;
 }
 
 // Initial frontier: every alive vertex of degree < k, stamped lvl = k.
@@ -650,10 +651,19 @@ __global__ __launch_bounds__(kBlock) void k_core_sweep_items(uint32_t* lvl, cons
 // wave leaves through one exit only.
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// With stop_at != 0 (the caller restores the level when the k-core is
+// empty), a wave that finds stop_at vertices queued -- so fewer than k + 1
+// can be left, and a non-empty k-core has at least k + 1 -- raises kVqEmpty,
+// and every wave leaves without finishing the cascade.
+__global__ void k_vcascade_seed(Ctl* ctl) {
+  if (threadIdx.x == 0) ctl->vq_removed = ctl->vq_reserve;  // the sweep queued only vertices
+}
+
 __global__ __launch_bounds__(kBlock) void k_core_cascade(const uint64_t* __restrict__ off,
                                                          const uint32_t* __restrict__ adj, uint32_t* lvl,
                                                          uint32_t* deg, uint32_t k, unsigned long long* q,
-                                                         uint32_t cap, Ctl* ctl, unsigned long long* stats) {
+                                                         uint32_t cap, Ctl* ctl, unsigned long long* stats,
+                                                         uint32_t stop_at) {
   const int lane = lane64();
   const unsigned long long below = (1ull << lane) - 1ull;
   uint32_t quit = 0;
@@ -676,6 +686,10 @@ __global__ __launch_bounds__(kBlock) void k_core_cascade(const uint64_t* __restr
         asm volatile("" : "+v"(z) : "v"(d));
         const uint32_t r = ld_coherent(&ctl->vq_reserve + z);
         st = err ? 2u : (h < r ? 1u : (d == r ? 2u : 0u));
+        if (st == 1u && stop_at && ld_coherent(&ctl->vq_removed) >= stop_at) {
+          atomicOr(&ctl->vq_err, kVqEmpty);
+          st = 2u;
+        }
       }
       st = uni(__shfl(st, 0));
       go = st == 1u;
@@ -703,7 +717,7 @@ __global__ __launch_bounds__(kBlock) void k_core_cascade(const uint64_t* __restr
         const unsigned long long x = ld_coherent(&q[h]);
         lo = uint32_t(x);
         hi = uint32_t(x >> 32);
-        if (x == kItemEmpty) err = ld_coherent(&ctl->vq_err);
+        if (x == kItemEmpty) err = ld_coherent(&ctl->vq_err) & (kVqOverflow | kVqStall);
       }
       ilo = uni(__shfl(lo, 0));
       ihi = uni(__shfl(hi, 0));
@@ -756,7 +770,10 @@ __global__ __launch_bounds__(kBlock) void k_core_cascade(const uint64_t* __restr
           const uint32_t leader = uni(uint32_t(__ffsll(m) - 1));
           const uint32_t cnt = uni(uint32_t(__popcll(m)));
           uint32_t base = 0;
-          if (uint32_t(lane) == leader) base = atomicAdd(&ctl->vq_reserve, cnt);
+          if (uint32_t(lane) == leader) {
+            base = atomicAdd(&ctl->vq_reserve, cnt);
+            if (stop_at) atomicAdd(&ctl->vq_removed, cnt);
+          }
           base = uni(__shfl(base, int(leader)));
           if (uint64_t(base) + cnt > cap) {
             if (lane == 0) atomicOr(&ctl->vq_err, kVqOverflow);
@@ -1964,11 +1981,12 @@ void core_round(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t* 
 uint64_t cascade_capacity(uint32_t n, uint64_t slots) { return uint64_t(n) + slots / kCascadeChunk + 2; }
 void core_cascade(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t k, uint32_t* lvl, uint32_t* deg,
                   unsigned long long* q, uint32_t cap, Ctl* ctl, unsigned long long* stats, hipEvent_t ev0,
-                  hipEvent_t ev1, hipStream_t s) {
+                  hipEvent_t ev1, hipStream_t s, uint32_t stop_at) {
   fill_u64(reinterpret_cast<uint64_t*>(q), cap, kItemEmpty, s);
   k_vcascade_start<<<1, kWave, 0, s>>>(ctl);
   if (!n) return;
   k_core_sweep_items<<<grid_tiles(n), kBlock, 0, s>>>(lvl, deg, n, k, q, ctl);
+  if (stop_at) k_vcascade_seed<<<1, kWave, 0, s>>>(ctl);
   // enough waves to keep a wide cascade busy; idle ones leave at quiescence
   const int grid = int(std::min<uint64_t>(1024, std::max<uint64_t>(8, (uint64_t(n) + 255) / 256)));
   if (ev0 && ev1) {  // measurement: events stamped at the kernel's own start and end
@@ -1981,11 +1999,12 @@ void core_cascade(const uint64_t* off, const uint32_t* adj, uint32_t n, uint32_t
     uint32_t a_cap = cap;
     Ctl* a_ctl = ctl;
     unsigned long long* a_stats = stats;
-    void* args[] = {&a_off, &a_adj, &a_lvl, &a_deg, &a_k, &a_q, &a_cap, &a_ctl, &a_stats};
+    uint32_t a_stop = stop_at;
+    void* args[] = {&a_off, &a_adj, &a_lvl, &a_deg, &a_k, &a_q, &a_cap, &a_ctl, &a_stats, &a_stop};
     KT_HIP(hipExtLaunchKernel(reinterpret_cast<const void*>(&k_core_cascade), dim3(grid), dim3(kBlock), args, 0, s,
                               ev0, ev1, 0));
   } else {
-    k_core_cascade<<<grid, kBlock, 0, s>>>(off, adj, lvl, deg, k, q, cap, ctl, stats);
+    k_core_cascade<<<grid, kBlock, 0, s>>>(off, adj, lvl, deg, k, q, cap, ctl, stats, stop_at);
   }
 }
 void core_alive_degrees(const uint64_t* off, const uint32_t* adj, const uint32_t* lvl, uint32_t n,

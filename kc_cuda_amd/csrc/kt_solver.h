@@ -197,8 +197,8 @@ class Solver {
 
   // stages (kt_stages.hip)
   void load_csr_host(uint32_t n, const uint64_t* off, const uint32_t* adj);
-  uint64_t peel_vertices(LevelGraph& g, uint32_t k);
-  uint64_t peel_vertices_rounds(LevelGraph& g, uint32_t k);
+  uint64_t peel_vertices(LevelGraph& g, uint32_t k, uint64_t alive = 0);
+  uint64_t peel_vertices_rounds(LevelGraph& g, uint32_t k, uint64_t stop_at, uint64_t alive);
   // vertex peel: 0 by level size (one cascade launch per threshold up to
   // kCascadeMaxSlots row slots, synchronous rounds above), 1 always rounds,
   // 2 always the cascade

@@ -47,10 +47,16 @@ constexpr uint32_t kMinCut = 256;  // smaller alive sets stay on their level
 // run on the device in batches with the host reading the control block
 // between batches, whose fire-and-forget decrements stream faster than the
 // cascade's returning ones (set_vertex_mode forces either).
-uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
+// With alive != 0 (the level's alive vertices, for a caller that restores
+// the level when the k-core is empty), the peel may stop as soon as fewer
+// than k + 1 vertices are left -- a non-empty k-core has at least k + 1 --
+// and then returns `alive` with the level part-peeled.
+uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k, uint64_t alive) {
   if (g.n == 0) return 0;
+  if (alive && alive <= k) return alive;  // no k-core fits
+  const uint64_t stop_at = alive ? alive - k : 0;  // removed >= stop_at: fewer than k + 1 left
   vertex_rounds_ = vertex_mode_ == 1 || (vertex_mode_ == 0 && g.slots > kCascadeMaxSlots);
-  if (vertex_rounds_) return peel_vertices_rounds(g, k);
+  if (vertex_rounds_) return peel_vertices_rounds(g, k, stop_at, alive);
   const uint64_t cap = cascade_capacity(g.n, g.slots);
   if (cap >= 0xFFFFFFFFull) fail("vertex peel: cascade queue exceeds 2^32 items");
   vq_.ensure(sizeof(unsigned long long) * cap);
@@ -65,16 +71,21 @@ uint64_t Solver::peel_vertices(LevelGraph& g, uint32_t k) {
     if (probe_push_ & 2) ps = probe_stats_.as<unsigned long long>();  // measurement: slot counters
   }
   core_cascade(g.off, g.adj, g.n, k, g.lvl, g.deg, vq_.as<unsigned long long>(), uint32_t(cap), ctl_, ps, e0, e1,
-               stream_);
+               stream_, uint32_t(std::min<uint64_t>(stop_at, 0xFFFFFFFFull)));
   const Ctl h = read_ctl();
-  if (h.vq_err) fail(h.vq_err & kVqOverflow ? "vertex peel: cascade queue overflow" : "vertex peel: cascade stalled");
+  if (h.vq_err & (kVqOverflow | kVqStall))
+    fail(h.vq_err & kVqOverflow ? "vertex peel: cascade queue overflow" : "vertex peel: cascade stalled");
+  if (h.vq_err & kVqEmpty) {
+    if (trace_) std::fprintf(stderr, "[kt] vertex-peel level_n=%u k=%u empty (stopped early)\n", g.n, k);
+    return alive;
+  }
   const uint64_t removed = uint64_t(h.vq_reserve) - h.vq_chunks;
   if (trace_) std::fprintf(stderr, "[kt] vertex-peel level_n=%u k=%u removed=%llu\n", g.n, k, (unsigned long long)removed);
   if (removed > g.n || h.vq_done != h.vq_reserve) fail("vertex peel bookkeeping mismatch");
   return removed;
 }
 
-uint64_t Solver::peel_vertices_rounds(LevelGraph& g, uint32_t k) {
+uint64_t Solver::peel_vertices_rounds(LevelGraph& g, uint32_t k, uint64_t stop_at, uint64_t alive) {
   constexpr int kMaxBatch = 32;
   if (g.n == 0) return 0;
   q0_.ensure(sizeof(uint32_t) * (size_t(g.n) + 1));
@@ -105,6 +116,7 @@ uint64_t Solver::peel_vertices_rounds(LevelGraph& g, uint32_t k) {
       if (h.removed > g.n) fail("vertex peel bookkeeping mismatch");
       return h.removed;
     }
+    if (stop_at && h.removed >= stop_at) return alive;  // fewer than k + 1 left: the k-core is empty
     if (queued > uint64_t(g.n) + 2 * kMaxBatch) fail("vertex peel did not converge");
     batch = std::min(batch * 2, kMaxBatch);
   }
@@ -280,7 +292,7 @@ uint32_t Solver::max_core(bool allow_cut, bool fresh) {
       return decompose_level(*g, k);
     const uint32_t kk = k + step;
     KT_HIP(hipMemcpyAsync(deg_bak_.p, g->deg, sizeof(uint32_t) * g->n, hipMemcpyDeviceToDevice, stream_));
-    const uint64_t removed = peel_vertices(*g, kk);
+    const uint64_t removed = peel_vertices(*g, kk, alive);  // an overshoot is restored below
     if (removed < alive) {
       alive -= removed;
       k = kk;
