@@ -1,9 +1,12 @@
-// Host side of the batched solve: validation, the union of the valid graphs,
-// one truss decomposition, per-graph answers.
+// Host side of the batched solve: validation, the unions of the valid graphs
+// (one per part of the batch), their truss decompositions, per-graph answers.
 #include "kt_batch_host.h"
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
+#include <exception>
+#include <thread>
 #include <vector>
 
 #include "kt_batch_union.h"
@@ -11,7 +14,10 @@
 
 namespace kt {
 
-BatchRunner::BatchRunner(Solver* s) : solver_(s) {}
+BatchRunner::BatchRunner(Solver* s, uint32_t max_parts) : solver_(s) {
+  if (const char* v = std::getenv("KT_BATCH_PARTS")) parts_wanted_ = uint32_t(std::max(1, std::min(16, std::atoi(v))));
+  if (max_parts) parts_wanted_ = max_parts;
+}
 
 BatchRunner::~BatchRunner() {
   (void)hipSetDevice(solver_->device());
@@ -21,12 +27,68 @@ BatchRunner::~BatchRunner() {
 
 uint64_t BatchRunner::device_bytes() const {
   uint64_t b = 0;
-  for (const DevBuf* d : {&poff_, &ns_, &status_, &vstart_, &upoff_, &union_, &gk_, &gcnt_}) b += d->bytes;
+  for (const DevBuf* d : {&poff_, &ns_, &status_, &gk_, &gcnt_}) b += d->bytes;
+  for (const Part& p : parts_) {
+    for (const DevBuf* d : {&p.poff, &p.vstart, &p.upoff, &p.uni}) b += d->bytes;
+    if (p.own) b += p.own->device_bytes();
+  }
   return b;
 }
 
 void BatchRunner::release_buffers() {
-  for (DevBuf* d : {&poff_, &ns_, &status_, &vstart_, &upoff_, &union_, &gk_, &gcnt_}) d->release();
+  for (DevBuf* d : {&poff_, &ns_, &status_, &gk_, &gcnt_}) d->release();
+  for (Part& p : parts_) {
+    if (p.own) (void)hipStreamSynchronize(p.own->stream());
+    for (DevBuf* d : {&p.poff, &p.vstart, &p.upoff, &p.uni}) d->release();
+  }
+  parts_.clear();  // their solvers free their own buffers
+}
+
+// Graphs [g0, g1) of the batch as one union on solver sv (and its stream):
+// union numbering of its valid graphs, the union, its truss decomposition
+// and the per-graph answers into gk_ / gcnt_ (and keep) at [g0, g1). The
+// part's pairs, offsets and results are slices of the batch's, so parts
+// write disjoint slots.
+void BatchRunner::solve_part(Part& part, Solver* sv, const uint32_t* d_pairs, const uint64_t* h_off,
+                             const uint32_t* h_ns, const std::vector<uint32_t>& st, uint32_t g0, uint32_t g1,
+                             uint8_t* d_keep) {
+  KT_HIP(hipSetDevice(sv->device()));
+  const hipStream_t s = sv->stream();
+  const uint32_t cnt = g1 - g0;
+  const uint64_t base = h_off[g0];
+  std::vector<uint64_t> poff(size_t(cnt) + 1), upoff(cnt);
+  std::vector<uint32_t> vstart(size_t(cnt) + 1);
+  uint64_t nv = 0, mu = 0;
+  for (uint32_t i = 0; i < cnt; ++i) {
+    const uint32_t g = g0 + i;
+    poff[i] = h_off[g] - base;
+    vstart[i] = uint32_t(nv);
+    upoff[i] = mu;
+    if (st[g]) continue;
+    nv += h_ns[g];
+    mu += h_off[g + 1] - h_off[g];
+    if (nv >= (uint64_t(1) << 32)) fail("batch: the union of the graphs has 2^32 or more vertices");
+  }
+  poff[cnt] = h_off[g1] - base;
+  vstart[cnt] = uint32_t(nv);
+  if (!mu) return;
+  part.poff.ensure(sizeof(uint64_t) * (size_t(cnt) + 1));
+  part.vstart.ensure(sizeof(uint32_t) * (size_t(cnt) + 1));
+  part.upoff.ensure(sizeof(uint64_t) * cnt);
+  part.uni.ensure(sizeof(uint32_t) * 2 * mu);
+  KT_HIP(hipMemcpyAsync(part.poff.p, poff.data(), sizeof(uint64_t) * (size_t(cnt) + 1), hipMemcpyHostToDevice, s));
+  KT_HIP(hipMemcpyAsync(part.vstart.p, vstart.data(), sizeof(uint32_t) * (size_t(cnt) + 1), hipMemcpyHostToDevice,
+                        s));
+  KT_HIP(hipMemcpyAsync(part.upoff.p, upoff.data(), sizeof(uint64_t) * cnt, hipMemcpyHostToDevice, s));
+  const uint32_t* pairs = d_pairs + 2 * base;
+  batch_union(pairs, part.poff.as<uint64_t>(), cnt, poff[cnt], part.vstart.as<uint32_t>(),
+              status_.as<uint32_t>() + g0, part.upoff.as<uint64_t>(), part.uni.as<uint32_t>(), s);
+  CoreView cv;
+  const uint32_t kall = sv->decompose_pairs_device(part.uni.as<uint32_t>(), mu, uint32_t(nv), &cv);
+  batch_truss(cv, kall, part.vstart.as<uint32_t>(), cnt, gk_.as<uint32_t>() + g0,
+              reinterpret_cast<unsigned long long*>(gcnt_.p) + g0, pairs, part.poff.as<uint64_t>(),
+              d_keep ? d_keep + base : nullptr, s);
+  KT_HIP(hipStreamSynchronize(s));
 }
 
 void BatchRunner::solve(const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns, uint32_t count,
@@ -131,37 +193,53 @@ void BatchRunner::solve_impl(const uint32_t* d_pairs, const uint64_t* h_off, con
     KT_HIP(hipMemcpyAsync(status_.p, st.data(), sizeof(uint32_t) * count, hipMemcpyHostToDevice, s));
   }
 
-  // union numbering of the valid graphs (an invalid graph takes no vertex)
-  std::vector<uint32_t> vstart(size_t(count) + 1);
-  std::vector<uint64_t> upoff(count);
-  uint64_t nv = 0, mu = 0;
-  for (uint32_t g = 0; g < count; ++g) {
-    vstart[g] = uint32_t(nv);
-    upoff[g] = mu;
-    if (st[g]) continue;
-    nv += h_ns[g];
-    mu += h_off[g + 1] - h_off[g];
-    if (nv >= (uint64_t(1) << 32)) fail("batch: the union of the graphs has 2^32 or more vertices");
-  }
-  vstart[count] = uint32_t(nv);
-  vstart_.ensure(sizeof(uint32_t) * (size_t(count) + 1));
-  upoff_.ensure(sizeof(uint64_t) * count);
   gk_.ensure(sizeof(uint32_t) * count);
   gcnt_.ensure(sizeof(unsigned long long) * count);
-  KT_HIP(hipMemcpyAsync(vstart_.p, vstart.data(), sizeof(uint32_t) * (size_t(count) + 1), hipMemcpyHostToDevice, s));
-  KT_HIP(hipMemcpyAsync(upoff_.p, upoff.data(), sizeof(uint64_t) * count, hipMemcpyHostToDevice, s));
   fill_u32(gk_.as<uint32_t>(), count, 0, s);
   fill_u64(gcnt_.as<uint64_t>(), count, 0, s);
   if (d_keep) fill_u8(d_keep, total, 0, s);
-  if (mu) {
-    union_.ensure(sizeof(uint32_t) * 2 * mu);
-    batch_union(d_pairs, poff_.as<uint64_t>(), count, total, vstart_.as<uint32_t>(), status_.as<uint32_t>(),
-                upoff_.as<uint64_t>(), union_.as<uint32_t>(), s);
-    CoreView cv;
-    const uint32_t kall = solver_->decompose_pairs_device(union_.as<uint32_t>(), mu, uint32_t(nv), &cv);
-    batch_truss(cv, kall, vstart_.as<uint32_t>(), count, gk_.as<uint32_t>(),
-                reinterpret_cast<unsigned long long*>(gcnt_.p), d_pairs, poff_.as<uint64_t>(), d_keep, s);
+  // parts: contiguous graph ranges of about equal valid edges, each one
+  // union decomposed on its own solver and stream, all at once (an invalid
+  // graph takes no vertex and no edge)
+  uint64_t valid = 0;
+  for (uint32_t g = 0; g < count; ++g)
+    if (!st[g]) valid += h_off[g + 1] - h_off[g];
+  const uint32_t np = valid < (uint64_t(1) << 16) ? 1u : std::max(1u, std::min(parts_wanted_, count));
+  std::vector<uint32_t> cut(size_t(np) + 1, count);
+  cut[0] = 0;
+  {
+    uint64_t acc = 0;
+    uint32_t p = 1;
+    for (uint32_t g = 0; g < count && p < np; ++g) {
+      if (!st[g]) acc += h_off[g + 1] - h_off[g];
+      if (acc * np >= valid * p) cut[p++] = g + 1;
+    }
   }
+  while (parts_.size() < np) parts_.emplace_back();
+  const Solver::Tuning tune = solver_->tuning();
+  for (uint32_t p = 1; p < np; ++p) {
+    if (!parts_[p].own) parts_[p].own.reset(new Solver(solver_->device()));
+    parts_[p].own->set_tuning(tune);
+  }
+  KT_HIP(hipStreamSynchronize(s));  // zeroed results and the status before any part reads them
+  std::vector<std::exception_ptr> errs(np);
+  std::vector<std::thread> threads;
+  for (uint32_t p = 1; p < np; ++p)
+    threads.emplace_back([&, p] {
+      try {
+        solve_part(parts_[p], parts_[p].own.get(), d_pairs, h_off, h_ns, st, cut[p], cut[p + 1], d_keep);
+      } catch (...) {
+        errs[p] = std::current_exception();
+      }
+    });
+  try {
+    solve_part(parts_[0], solver_, d_pairs, h_off, h_ns, st, cut[0], cut[1], d_keep);
+  } catch (...) {
+    errs[0] = std::current_exception();
+  }
+  for (std::thread& t : threads) t.join();
+  for (const std::exception_ptr& e : errs)
+    if (e) std::rethrow_exception(e);
   batch_results(gk_.as<uint32_t>(), reinterpret_cast<const unsigned long long*>(gcnt_.p), status_.as<uint32_t>(),
                 count, d_kmax, reinterpret_cast<unsigned long long*>(d_count), d_status, s);
   KT_HIP(hipStreamSynchronize(s));

@@ -1,7 +1,8 @@
 // Host orchestration of the batched solve (one per context): the valid graphs
-// of a batch become one disjoint union whose truss decomposition the
-// context's solver computes; per-graph answers are read off it.
+// of a batch become disjoint unions (one per part of the batch, decomposed
+// concurrently) whose truss decompositions give the per-graph answers.
 #pragma once
+#include <memory>
 #include <vector>
 
 #include "kt_solver.h"
@@ -17,7 +18,9 @@ struct BatchTuning {
 
 class BatchRunner {
  public:
-  explicit BatchRunner(Solver* s);
+  // max_parts: concurrent parts a batch may be split into (0: KT_BATCH_PARTS
+  // or 4; serving slots pass 1, their groups sharing the slot's solver only)
+  explicit BatchRunner(Solver* s, uint32_t max_parts = 0);
   ~BatchRunner();
   // d_pairs: all graphs' canonical pairs (sorted within each graph) in HBM;
   // h_off / h_ns: host offsets (count + 1) and vertex counts. Results go to
@@ -33,16 +36,28 @@ class BatchRunner {
   // into the outputs.
   BatchTuning autotune(const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns, uint32_t count,
                        uint32_t every, uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status);
-  // device bytes of its own buffers (the solver's are counted by the solver)
+  // device bytes of its own buffers and of its extra parts' solvers (the
+  // context solver's are counted by the solver)
   uint64_t device_bytes() const;
   void release_buffers();
 
  private:
+  // A part of the batch: a contiguous range of graphs whose union one
+  // solver (the context's for part 0, an own one with its own stream for
+  // the others) decomposes; the parts run concurrently.
+  struct Part {
+    std::unique_ptr<Solver> own;
+    DevBuf poff, vstart, upoff, uni;
+  };
   // every > 1: graphs g with g % every != 0 take no part (status 2)
   void solve_impl(const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns, uint32_t count,
                   uint32_t* d_kmax, uint64_t* d_count, uint32_t* d_status, uint8_t* d_keep, uint32_t every);
+  void solve_part(Part& part, Solver* sv, const uint32_t* d_pairs, const uint64_t* h_off, const uint32_t* h_ns,
+                  const std::vector<uint32_t>& st, uint32_t g0, uint32_t g1, uint8_t* d_keep);
   Solver* solver_;
-  DevBuf poff_, ns_, status_, vstart_, upoff_, union_, gk_, gcnt_;
+  DevBuf poff_, ns_, status_, gk_, gcnt_;
+  std::vector<Part> parts_;
+  uint32_t parts_wanted_ = 4;  // concurrent parts of a batch (1: one union)
 };
 
 }  // namespace kt

@@ -35,7 +35,7 @@ Server::Server(int device, int slots, uint64_t budget) : device_(device), budget
   for (auto& s : slots_) {
     s.solver = std::make_unique<Solver>(device_);
     s.solver->set_eidm_budget(kServeEidmBytes);
-    s.batch = std::make_unique<BatchRunner>(s.solver.get());
+    s.batch = std::make_unique<BatchRunner>(s.solver.get(), 1);
     s.held = s.charge = slot_bytes(s);
     ctr_.bytes_in_use += s.charge;
   }
