@@ -164,9 +164,12 @@ int kt_set_tuning(kt_context* ctx, int count_lanes, int peel_lanes, int flags);
  * (vertex counts) are host arrays. Per graph: d_kmax, d_count (edges in its
  * kmax-truss) and d_status (0 ok, 1 invalid input); d_keep (nullable) gets 1
  * for every pair in its graph's kmax-truss. The valid graphs are solved
- * together as one disjoint union: one truss decomposition of the union on
- * this context, from which every graph's answer is read. Results equal
- * kt_solve_pairs_* on each graph. flags is reserved and must be 0. */
+ * together as disjoint unions: the batch is cut into up to four contiguous
+ * parts of about equal edges (KT_BATCH_PARTS sets the count), and each
+ * part's union is decomposed on its own solver and stream (the context's
+ * for the first), all at once; every graph's answer is read off its part's
+ * decomposition. Results equal kt_solve_pairs_* on each graph. flags is
+ * reserved and must be 0. */
 int kt_batch_solve(kt_context* ctx, const uint32_t* d_pairs, const uint64_t* h_offsets,
                    const uint32_t* h_ns, uint32_t count, uint32_t* d_kmax, uint64_t* d_count,
                    uint32_t* d_status, uint8_t* d_keep, int flags);
