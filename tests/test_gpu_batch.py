@@ -1,6 +1,9 @@
-"""Batched solver (many graphs solved as one disjoint union): every graph's
-kmax, |T| and kmax-truss edge set equal the single-graph path's; invalid
-graphs are flagged without disturbing the others; repeated solves agree."""
+"""Batched solver (many graphs solved as disjoint unions, one per part of
+the batch): every graph's kmax, |T| and kmax-truss edge set equal the
+single-graph path's; invalid graphs are flagged without disturbing the
+others; repeated solves agree; the number of parts changes no answer."""
+import os
+
 import numpy as np
 import pytest
 
@@ -153,3 +156,62 @@ def test_batch_empty_and_all_invalid(ctx):
         assert (r.status == 1).all() and (r.kmax == 0).all() and not r.keep.any()
     finally:
         d.free()
+
+
+@pytest.mark.parametrize("parts", ["1", "3", "7"])
+def test_batch_parts_agree(parts):
+    """The batch cut into 1, 3 or 7 concurrent parts (KT_BATCH_PARTS, each
+    part's union on its own solver and stream): every graph's kmax, |T|,
+    status and kept pairs equal the oracle's, for a mix of R-MAT / ER graphs,
+    the closed forms (empty, triangle-free, cliques) and invalid graphs
+    spread through the batch, so parts hold invalid and empty graphs."""
+    from kc_cuda_amd import Context, gen
+    from oracle import oracle as O
+    pairs, offs, ns = gen.batch(40, 16, 6000, 6.0, 20.0, 3, 16, 31)
+    pairs, offs, ns = _with_specials(pairs, offs, ns)
+    # an invalid graph after every 9th valid one
+    ps, os_, nn, bad = [], [0], [], []
+    total = 0
+    for g in range(len(ns)):
+        q = pairs[int(offs[g]):int(offs[g + 1])]
+        ps.append(q)
+        total += len(q)
+        os_.append(total)
+        nn.append(int(ns[g]))
+        if g % 9 == 8:
+            ps.append(np.array([[2, 1], [0, 3]], np.uint32))  # not canonical
+            total += 2
+            os_.append(total)
+            nn.append(4)
+            bad.append(len(nn) - 1)
+    pairs = np.concatenate(ps).astype(np.uint32)
+    offs = np.array(os_, dtype=np.uint64)
+    ns = np.array(nn, dtype=np.uint32)
+    assert len(pairs) >= (1 << 16), "the batch must be large enough to be cut into parts"
+    old = os.environ.get("KT_BATCH_PARTS")
+    os.environ["KT_BATCH_PARTS"] = parts
+    try:
+        c = Context(0)  # the batch runner reads the knob when it is made
+    finally:
+        if old is None:
+            os.environ.pop("KT_BATCH_PARTS")
+        else:
+            os.environ["KT_BATCH_PARTS"] = old
+    try:
+        print(f"[gpu] batch parts={parts} graphs={len(ns)} pairs={len(pairs)}", flush=True)
+        res = _solve_batch(c, pairs, offs, ns, keep=True)
+        want_bad = np.zeros(len(ns), dtype=bool)
+        want_bad[bad] = True
+        np.testing.assert_array_equal(res.status != 0, want_bad)
+        for g in range(len(ns)):
+            q = pairs[int(offs[g]):int(offs[g + 1])]
+            if want_bad[g]:
+                assert (int(res.kmax[g]), int(res.edges[g])) == (0, 0)
+                assert not res.keep[int(offs[g]):int(offs[g + 1])].any()
+                continue
+            ref = O.solve_pairs(q, int(ns[g]), want_edges=True)
+            assert (int(res.kmax[g]), int(res.edges[g])) == (ref.kmax, ref.truss_edges), f"graph {g}"
+            kept = q[res.keep[int(offs[g]):int(offs[g + 1])] == 1]
+            np.testing.assert_array_equal(canon_pairs(kept), canon_pairs(ref.truss), err_msg=f"graph {g}")
+    finally:
+        c.close()
