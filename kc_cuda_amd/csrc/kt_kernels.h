@@ -136,7 +136,14 @@ void alive_ranked_list(const uint32_t* bits, const uint32_t* pref, uint32_t n, u
 // + 1 suffices)
 void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* abits,
               const uint32_t* apref, const uint32_t* acoarse, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj,
-              HeavyList hl, uint64_t* cursor, uint32_t cursor_cap, hipStream_t s);
+              HeavyList hl, uint64_t* cursor, uint32_t cursor_cap, hipStream_t s, uint32_t* sub_deg = nullptr);
+// The cut without a counting pass: len[0..na] = the listed rows' lengths
+// (len[na] = 0), scanned into upper-bound offsets for sub_fill (with
+// sub_deg: it then counts each row's alive neighbours), then rows_pack
+// moves the rows to their exact offsets.
+void list_row_lengths(const uint64_t* off, const uint32_t* list, uint32_t na, uint32_t* len, hipStream_t s);
+void rows_pack(const uint32_t* tmp, const uint64_t* ub_off, const uint64_t* off, const uint32_t* deg, uint32_t na,
+               uint32_t* adj, hipStream_t s);
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to,
                   const uint32_t* inv, uint32_t nc, uint32_t* dout, hipStream_t s);
 void orient_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* to,

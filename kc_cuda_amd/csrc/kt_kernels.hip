@@ -1278,6 +1278,8 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill_heavy(const uint64_t* __res
   }
 }
 
+// sub_deg (nullable): the row's alive-neighbour count, for rows laid out
+// at an upper bound of their length (the cut without a counting pass)
 __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict__ off,
                                                      const uint32_t* __restrict__ adj,
                                                      const uint32_t* __restrict__ list,
@@ -1286,7 +1288,7 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict_
                                                      const uint32_t* __restrict__ acoarse, uint32_t na,
                                                      const uint64_t* __restrict__ sub_off,
                                                      uint32_t* sub_adj, uint32_t* heavy,
-                                                     uint32_t* heavy_count) {
+                                                     uint32_t* heavy_count, uint32_t* sub_deg) {
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   const int lane = lane64();
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < na; i += waves) {
@@ -1308,6 +1310,38 @@ __global__ __launch_bounds__(kBlock) void k_sub_fill(const uint64_t* __restrict_
         pos += uint32_t(__popcll(mask));
       }
     }
+    if (sub_deg && lane == 0) sub_deg[i] = uint32_t(pos - sub_off[i]);
+  }
+}
+
+// A heavy row's alive-neighbour count is its fill cursor.
+__global__ void k_heavy_counts(const uint32_t* heavy, const uint32_t* heavy_count,
+                               const unsigned long long* __restrict__ cursor, uint32_t* sub_deg) {
+  const uint32_t hc = *heavy_count;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < hc; h += gridDim.x * blockDim.x)
+    sub_deg[heavy[h]] = uint32_t(cursor[h]);
+}
+
+// len[i] = the row length of list[i] (an upper bound of its alive part).
+__global__ void k_list_row_len(const uint64_t* __restrict__ off, const uint32_t* __restrict__ list, uint32_t na,
+                               uint32_t* len) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= na; i += gridDim.x * blockDim.x)
+    len[i] = i < na ? uint32_t(off[list[i] + 1] - off[list[i]]) : 0u;
+}
+
+// Rows filled at an upper bound of their length, packed: one wave per row
+// copies its deg[i] entries from tmp[ub_off[i]] to adj[off[i]].
+__global__ __launch_bounds__(kBlock) void k_rows_pack(const uint32_t* __restrict__ tmp,
+                                                      const uint64_t* __restrict__ ub_off,
+                                                      const uint64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ deg, uint32_t na,
+                                                      uint32_t* __restrict__ adj) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const int lane = lane64();
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < na; i += waves) {
+    const uint64_t from = ub_off[i], to = off[i];
+    const uint32_t d = deg[i];
+    for (uint32_t j = uint32_t(lane); j < d; j += kWave) adj[to + j] = tmp[from + j];
   }
 }
 
@@ -2082,15 +2116,25 @@ void alive_ranked_list(const uint32_t* bits, const uint32_t* pref, uint32_t n, u
 }
 void sub_fill(const uint64_t* off, const uint32_t* adj, const uint32_t* list, const uint32_t* abits,
               const uint32_t* apref, const uint32_t* acoarse, uint32_t na, const uint64_t* sub_off, uint32_t* sub_adj,
-              HeavyList hl, uint64_t* cursor, uint32_t cursor_cap, hipStream_t s) {
+              HeavyList hl, uint64_t* cursor, uint32_t cursor_cap, hipStream_t s, uint32_t* sub_deg) {
   if (!na) return;
   fill_u32(hl.count, 1, 0, s);
   fill_u64(cursor, cursor_cap, 0, s);
   k_sub_fill<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(off, adj, list, abits, apref, acoarse, na, sub_off,
-                                                               sub_adj, hl.items, hl.count);
+                                                               sub_adj, hl.items, hl.count, sub_deg);
   k_heavy_plan<<<1, kBlock, 0, s>>>(off, hl.items, hl.count, list, hl.slices, nullptr);
   k_sub_fill_heavy<<<kHeavyGrid, kBlock, 0, s>>>(off, adj, list, abits, apref, acoarse, sub_off, sub_adj, hl.items,
                                                  hl.count, hl.slices, reinterpret_cast<unsigned long long*>(cursor));
+  if (sub_deg)
+    k_heavy_counts<<<grid_for(cursor_cap), kBlock, 0, s>>>(hl.items, hl.count,
+                                                          reinterpret_cast<const unsigned long long*>(cursor), sub_deg);
+}
+void list_row_lengths(const uint64_t* off, const uint32_t* list, uint32_t na, uint32_t* len, hipStream_t s) {
+  k_list_row_len<<<grid_for(uint64_t(na) + 1), kBlock, 0, s>>>(off, list, na, len);
+}
+void rows_pack(const uint32_t* tmp, const uint64_t* ub_off, const uint64_t* off, const uint32_t* deg, uint32_t na,
+               uint32_t* adj, hipStream_t s) {
+  if (na) k_rows_pack<<<grid_for(uint64_t(na) * kWave), kBlock, 0, s>>>(tmp, ub_off, off, deg, na, adj);
 }
 void orient_count(const uint64_t* off, const uint32_t* adj, const uint32_t* to, const uint32_t* inv,
                   uint32_t nc, uint32_t* dout, hipStream_t s) {

@@ -154,16 +154,39 @@ LevelGraph& Solver::cut_level(LevelGraph& p, uint32_t k_enter, uint32_t alive, b
   s.ok.assign(1, k_enter);
   s.exact_from = s.exact_to = 0;
   fill_u32(s.deg + na, 1, 0, stream_);
-  sub_degrees(list_.as<uint32_t>(), recount ? nullptr : p.deg, p.to_orig, na, s.deg, s.to_orig, stream_);
-  if (recount) list_alive_degrees(p.off, p.adj, list_.as<uint32_t>(), na, abits, acoarse, s.deg, heavy(), stream_);
-  exclusive_scan_u32_to_u64(s.deg, s.off, na + 1);
-  s.slots = read_u64(s.off + na);
-  s.b_adj.ensure(sizeof(uint32_t) * std::max<uint64_t>(s.slots, 1));
-  s.adj = s.b_adj.as<uint32_t>();
+  const uint32_t* list = list_.as<uint32_t>();
+  sub_degrees(list, recount ? nullptr : p.deg, p.to_orig, na, s.deg, s.to_orig, stream_);
   const uint32_t cap = uint32_t(std::min<uint64_t>(na, p.slots / 8192 + 1));
   cursor_.ensure(sizeof(uint64_t) * (size_t(cap) + 1));
-  sub_fill(p.off, p.adj, list_.as<uint32_t>(), abits, apref, acoarse, na, s.off, s.adj, heavy(), cursor_.as<uint64_t>(), cap,
-           stream_);
+  // Degrees unknown and the build's partition buffer (free after the build)
+  // large enough for every kept row: fill the rows at their full lengths in
+  // it while counting their alive neighbours, then pack them (one walk of
+  // the kept rows instead of a counting walk and a filling walk).
+  const bool one_walk = recount && part_.bytes >= sizeof(uint32_t) * std::max<uint64_t>(p.slots, 1);
+  if (trace_)
+    std::fprintf(stderr, "[kt] cut level_n=%u -> %u recount=%d fill=%s\n", p.n, na, int(recount),
+                 one_walk ? "one-walk" : recount ? "count+fill" : "fill");
+  if (one_walk) {
+    keys_.ensure(sizeof(uint64_t) * (size_t(na) + 1));
+    uint64_t* ub_off = keys_.as<uint64_t>();
+    uint32_t* ub_adj = part_.as<uint32_t>();
+    list_row_lengths(p.off, list, na, s.deg, stream_);
+    exclusive_scan_u32_to_u64(s.deg, ub_off, na + 1);
+    sub_fill(p.off, p.adj, list, abits, apref, acoarse, na, ub_off, ub_adj, heavy(), cursor_.as<uint64_t>(), cap,
+             stream_, s.deg);
+    exclusive_scan_u32_to_u64(s.deg, s.off, na + 1);
+    s.slots = read_u64(s.off + na);
+    s.b_adj.ensure(sizeof(uint32_t) * std::max<uint64_t>(s.slots, 1));
+    s.adj = s.b_adj.as<uint32_t>();
+    rows_pack(ub_adj, ub_off, s.off, s.deg, na, s.adj, stream_);
+  } else {
+    if (recount) list_alive_degrees(p.off, p.adj, list, na, abits, acoarse, s.deg, heavy(), stream_);
+    exclusive_scan_u32_to_u64(s.deg, s.off, na + 1);
+    s.slots = read_u64(s.off + na);
+    s.b_adj.ensure(sizeof(uint32_t) * std::max<uint64_t>(s.slots, 1));
+    s.adj = s.b_adj.as<uint32_t>();
+    sub_fill(p.off, p.adj, list, abits, apref, acoarse, na, s.off, s.adj, heavy(), cursor_.as<uint64_t>(), cap, stream_);
+  }
   fill_u32(s.lvl, na, kNone, stream_);
   ++depth_;
   return s;

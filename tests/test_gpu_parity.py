@@ -166,6 +166,44 @@ def test_compaction_runs_on_rmat16(ctx):
     np.testing.assert_array_equal(ctx.truss_edges(), canon_pairs(ref.truss))
 
 
+@pytest.mark.parametrize("atomic", [False, True])
+def test_filter_cut_paths(capfd, atomic):
+    """The degree filter's cut of R-MAT 16 (h-index 656: the vertices of
+    degree >= 41 are kept, their alive neighbours counted from their rows).
+    After the bucketed build its partition buffer holds every kept row, so
+    the rows are filled at full length while being counted, then packed (one
+    walk); on a fresh context with the atomic build there is no such buffer
+    and the cut counts, then fills (two walks). Both equal the oracle."""
+    from kc_cuda_amd import Context, gen
+    pairs, n = gen.rmat(16)
+    ref = _oracle().solve_pairs(pairs, n, want_edges=True)
+    old = os.environ.get("KT_TRACE")
+    os.environ["KT_TRACE"] = "1"  # read when the context is made
+    try:
+        c = Context(0)
+    finally:
+        if old is None:
+            os.environ.pop("KT_TRACE")
+        else:
+            os.environ["KT_TRACE"] = old
+    try:
+        if atomic:
+            c.tune(force_atomic_build=True)
+        capfd.readouterr()
+        r = c.solve_pairs(pairs, n)
+        edges = c.truss_edges()
+        err = capfd.readouterr().err
+    finally:
+        c.close()
+    cuts = [l for l in err.splitlines() if l.startswith("[kt] cut ") and "recount=1" in l]
+    print(f"[gpu] filter cut atomic={atomic}: {cuts}", flush=True)
+    assert cuts, "the degree filter did not cut"
+    want = "fill=count+fill" if atomic else "fill=one-walk"
+    assert all(want in l for l in cuts)
+    assert (r.kmax, r.truss_edges) == (ref.kmax, ref.truss_edges)
+    np.testing.assert_array_equal(edges, canon_pairs(ref.truss))
+
+
 @pytest.mark.parametrize("build", ["auto", "atomic", "unsorted"])
 @pytest.mark.parametrize("order", ["sorted", "shuffled", "flipped"])
 @pytest.mark.parametrize("lines", [False, True])
