@@ -91,7 +91,7 @@ constexpr uint32_t kErrPeelCount = 1u << 8;
 constexpr uint32_t kRoundIdle = 0;     // vertex peel: nothing left to remove
 constexpr uint32_t kRoundPeel = 1;     // walk the frontier's triangles
 constexpr uint32_t kRoundDrop = 2;     // drop the frontier, recount the rest
-constexpr uint32_t kRoundEmpty = 3;    // the frontier is every live edge
+constexpr uint32_t kRoundEmpty = 3;    // the level empties: every live edge dies now
 constexpr uint32_t kRoundCompact = 4;  // compact the edge set first (unstamped)
 constexpr uint32_t kRoundMinSup = 5;   // level done: reduce the live supports
 constexpr uint32_t kRoundSweep = 6;    // next level: collect its frontier

@@ -41,8 +41,14 @@ constexpr int kClearGrid = 512;
 __device__ __forceinline__ uint32_t decide_round(uint32_t c, unsigned long long alive, int32_t thr,
                                                  const RoundRule& rule, bool allow_compact) {
   if (c == 0) return kRoundMinSup;  // the level is complete
+  // A non-empty level at thr is a (thr + 2)-truss: each of its vertices has
+  // at least thr + 1 neighbours on it, so it has at least
+  // (thr + 2)(thr + 1) / 2 edges. With fewer live edges left after this
+  // frontier (none: the frontier is every live edge) the level empties:
+  // every live edge dies at thr now.
+  if (c == alive || (thr > 0 && alive - c < (unsigned long long)(thr + 2) * (unsigned long long)(thr + 1) / 2))
+    return kRoundEmpty;
   if (allow_compact && 2ull * alive <= rule.mc) return kRoundCompact;
-  if (c == alive) return kRoundEmpty;
   const bool drop = rule.dense ? double(c) * (double(rule.ws) + rule.tri_cost * double(thr)) >=
                                      double(alive) * double(rule.ws) + rule.drop_fixed
                                : double(c) * rule.drop_ratio >= double(alive);
@@ -151,6 +157,17 @@ __device__ __forceinline__ void stamp_edge(uint32_t e, uint32_t r, uint32_t thr,
   }
 }
 
+// An emptied level (kRoundEmpty): every live edge of the edge set, the
+// frontier included, dies in round r at thr.
+__device__ __forceinline__ void stamp_all_live(uint32_t mc, uint32_t r, uint32_t thr, uint32_t* state,
+                                               const uint32_t* __restrict__ orig, uint32_t* del_thr) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < mc; e += gridDim.x * blockDim.x)
+    if (state[e] == kNone) {
+      state[e] = r;
+      del_thr[orig[e]] = thr;
+    }
+}
+
 // Stamp the first frontier of a search once (the rounds queued after a drop
 // or an empty level find it stamped already); with an empty frontier, the
 // level end instead.
@@ -170,10 +187,13 @@ __global__ __launch_bounds__(kBlock) void k_round_stamp(uint32_t* q0, uint32_t* 
   const uint32_t thr = uint32_t(ctl->thr);
   const uint32_t* q = cur ? q1 : q0;
   const bool fb = fbits && a == kRoundPeel;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
-    stamp_edge(q[i], r, thr, state, orig, del_thr, src, dst, fb ? fbits : nullptr, ws);
+  if (a == kRoundEmpty)
+    stamp_all_live(rule.mc, r, thr, state, orig, del_thr);
+  else
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+      stamp_edge(q[i], r, thr, state, orig, del_thr, src, dst, fb ? fbits : nullptr, ws);
   if (last_block(&ctl->ticket) && threadIdx.x == 0) {
-    ctl->alive -= cnt;
+    ctl->alive = a == kRoundEmpty ? 0 : ctl->alive - cnt;
     ctl->rounds_done += 1;
     ctl->stamped = 1;
   }
@@ -255,13 +275,16 @@ __global__ __launch_bounds__(kBlock) void k_round_next(uint32_t* q0, uint32_t* q
     const uint32_t rs = walked ? r + 1 : r;
     const uint32_t* q = nq ? q1 : q0;
     unsigned long long* fb = d == kRoundPeel ? fbits : nullptr;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
-      stamp_edge(q[i], rs, uint32_t(thr), state, orig, del_thr, src, dst, fb, ws);
+    if (d == kRoundEmpty)
+      stamp_all_live(rule.mc, rs, uint32_t(thr), state, orig, del_thr);
+    else
+      for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
+        stamp_edge(q[i], rs, uint32_t(thr), state, orig, del_thr, src, dst, fb, ws);
   }
   if (last_block(&ctl->ticket) && threadIdx.x == 0) {
     switch_queue();
     if (stamp) {
-      ctl->alive = alive - c;
+      ctl->alive = d == kRoundEmpty ? 0 : alive - c;
       ctl->rounds_done += 1;
     }
     ctl->stamped = stamp ? 1u : 0u;
